@@ -1,0 +1,570 @@
+// sfx_adjoint.hip -- exact Jacobian products of the SCF residual.
+//
+// Tangent (forward-mode) propagators: with g_t = exp(-u_t) and a direction v,
+//   G(1) = g,             dG(1) = -v g
+//   G(s) = g A[G(s-1)],   dG(s) = g (A[dG(s-1)] - v A[G(s-1)])
+// and the same for the backward propagator, where A is the lattice stencil of
+// the reference's PropagateG. The composition phi = pref * sum G Ginv / g gives
+//   dphi = phi (dpref/pref + v) + pref * sum (dG Ginv + G dGinv) / g,
+// dpref/pref = -dlnG_N for restricted chains (theta / (N G_N)), 0 for free
+// chains, dphi_b,solvent / phi_b,solvent for the solvent closure. The residual
+// of SF_Solve.cpp:554-590 is then linearised site by site. Every product is
+// exact to rounding; there is no finite-difference step anywhere.
+//
+// The transpose product J^T w runs the same recursions backwards (adjoint
+// propagators with the transposed stencil, applied as a scatter).
+#include <cmath>
+
+#include "sfx_device.h"
+#include "sfx_solve.h"
+
+namespace sfx {
+
+constexpr int kTB = 256;
+
+struct Tangent {
+  long long M = 0;
+  int maxstore = 0;
+  DevBuf<double> dstore, dpp, dacc, dphi_m, dphi, drho, dphitot, dmol, v64, v32, partial2;
+  DevBuf<double> adj_a, adj_b;  // VJP work arrays
+  void init(System& S) {
+    M = S.lat.d.M;
+    maxstore = S.maxstore;
+    dstore.alloc((size_t)maxstore * M);
+    dpp.alloc(2 * (size_t)M);
+    dacc.alloc((size_t)S.nm * S.ns * M);
+    dphi_m.alloc((size_t)S.nm * S.ns * M);
+    dphi.alloc((size_t)S.ns * M);
+    drho.alloc((size_t)S.ns * M);
+    dphitot.alloc((size_t)M);
+    dmol.alloc((size_t)S.nm * 4);
+    v64.alloc((size_t)S.ns * M);
+    v32.alloc((size_t)S.ns * M);  // per-state FP32 directions (half used)
+    partial2.alloc((size_t)S.npart);
+  }
+};
+
+template <typename T>
+static __global__ void k_zero(T* p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = T(0);
+}
+template <typename T>
+static void zero(T* p, long long n, hipStream_t st) {
+  k_zero<T><<<grid1(n, kTB), kTB, 0, st>>>(p, n);
+}
+
+static __global__ void k_to_f32(const double* a, float* b, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+template <typename T>
+__device__ __forceinline__ T vdir(const DevLattice& L, const T* v, int x, int y, int z) {
+  return v[sidx(L, x, y, z)];
+}
+
+// rank 1: G = g, dG = -v g
+template <typename T>
+__global__ void k_tinit(DevLattice L, const T* __restrict__ g, const T* __restrict__ v, T* __restrict__ G,
+                        T* __restrict__ dG, RankScale sc) {
+  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  float m = 0.f;
+  if (q < L.V) {
+    int x, y, z;
+    interior_coords(L, q, x, y, z);
+    long long r = sidx(L, x, y, z);
+    T gv = g[r];
+    G[r] = gv;
+    dG[r] = -v[r] * gv;
+    m = (float)gv;
+  }
+  if (sc.maxv) {
+    __shared__ float sm[kTB / 64];
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmaxf(m, sm[i]);
+      atomicMax(sc.maxv + sc.step, __float_as_uint(m));
+      if (blockIdx.x == 0) sc.lnv[sc.step] = 0.0;
+    }
+  }
+}
+
+// G' = g A[G] s, dG' = g (A[dG] - v A[G]) s; optional composition of
+// acc += G' Gp c, dacc += (dG' Gp + G' dGp) c
+template <typename T, int KIND, bool SCALED, bool CONNECT>
+__global__ void __launch_bounds__(kTB) k_tprop(DevLattice L, const T* __restrict__ Gin, const T* __restrict__ dGin,
+                                               const T* __restrict__ g, const T* __restrict__ v,
+                                               T* __restrict__ Gout, T* __restrict__ dGout,
+                                               const T* __restrict__ Gp, const T* __restrict__ dGp,
+                                               T* __restrict__ dacc, RankScale sc) {
+  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  T scale = T(1), cfac = T(1);
+  double lncum = 0;
+  if constexpr (SCALED) {
+    float mprev = __uint_as_float(sc.maxv[sc.step - 1]);
+    if (!(mprev > 0.f)) mprev = 1.f;
+    lncum = sc.lnv[sc.step - 1] + log((double)mprev);
+    scale = (T)(1.f / mprev);
+    if constexpr (CONNECT) cfac = (T)exp(lncum + sc.lnpair[sc.pair] - *sc.ref);
+  }
+  float m = 0.f;
+  if (q < L.V) {
+    int x, y, z;
+    interior_coords(L, q, x, y, z);
+    long long r = sidx(L, x, y, z);
+    T a = stencil<T, KIND>(L, Gin, x, y, z);
+    T da = stencil<T, KIND>(L, dGin, x, y, z);
+    T gv = g[r];
+    T Go = gv * a * scale;
+    T dGo = gv * (da - v[r] * a) * scale;
+    Gout[r] = Go;
+    dGout[r] = dGo;
+    if constexpr (CONNECT) dacc[r] += (dGo * Gp[r] + Go * dGp[r]) * cfac;
+    m = (float)Go;
+  }
+  if constexpr (SCALED) {
+    __shared__ float sm[kTB / 64];
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmaxf(m, sm[i]);
+      atomicMax(sc.maxv + sc.step, __float_as_uint(m));
+      if (blockIdx.x == 0) sc.lnv[sc.step] = lncum;
+    }
+  }
+}
+
+// dacc += (dA B + A dB) * half * c at interior sites
+template <typename T, bool SCALED>
+__global__ void k_tconnect(DevLattice L, const T* __restrict__ A, const T* __restrict__ dA, const T* __restrict__ B,
+                           const T* __restrict__ dB, T* __restrict__ dacc, RankScale sc, double half) {
+  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= L.V) return;
+  T c = T(half);
+  if constexpr (SCALED) c = (T)(half * exp(sc.lnv[sc.step] + sc.lnpair[sc.pair] - *sc.ref));
+  int x, y, z;
+  interior_coords(L, q, x, y, z);
+  long long r = sidx(L, x, y, z);
+  dacc[r] += (dA[r] * B[r] + A[r] * dB[r]) * c;
+}
+
+// two weighted sums at once: sum w G and sum w dG (per block)
+template <typename T>
+__global__ void k_wsum2(DevLattice L, const T* __restrict__ G, const T* __restrict__ dG, double* __restrict__ p1,
+                        double* __restrict__ p2) {
+  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0, b = 0;
+  if (q < L.V) {
+    int x, y, z;
+    interior_coords(L, q, x, y, z);
+    long long r = sidx(L, x, y, z);
+    double w = lngn_weight(L, x, y, z);
+    a = (double)G[r] * w;
+    b = (double)dG[r] * w;
+  }
+  __shared__ double sa[kTB / 64], sb[kTB / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = a;
+    sb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double A = 0, B = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+      A += sa[i];
+      B += sb[i];
+    }
+    p1[blockIdx.x] = A;
+    p2[blockIdx.x] = B;
+  }
+}
+
+// monomer: sum w g and sum w (-v g)
+__global__ void k_wsum_mono(DevLattice L, const double* __restrict__ g, const double* __restrict__ v,
+                            double* __restrict__ p1, double* __restrict__ p2) {
+  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0, b = 0;
+  if (q < L.V) {
+    int x, y, z;
+    interior_coords(L, q, x, y, z);
+    long long r = sidx(L, x, y, z);
+    double w = lngn_weight(L, x, y, z);
+    a = g[r] * w;
+    b = -v[r] * g[r] * w;
+  }
+  __shared__ double sa[kTB / 64], sb[kTB / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    b += __shfl_xor(b, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = a;
+    sb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double A = 0, B = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+      A += sa[i];
+      B += sb[i];
+    }
+    p1[blockIdx.x] = A;
+    p2[blockIdx.x] = B;
+  }
+}
+
+// tangent molecule scalars: dmol[0] = dlnGN, [1] = dpref/pref, [2] = dphib
+__global__ void k_tmol_finalize(MolParams mp, const double* p1, const double* p2, int n, const double* molscal,
+                                double* dmol, int nmol) {
+  if (threadIdx.x || blockIdx.x) return;
+  double* d = dmol + mp.index * 4;
+  double s1 = 0, s2 = 0;
+  for (int i = 0; i < n; i++) {
+    s1 += p1[i];
+    s2 += p2[i];
+  }
+  double dln = (p1 && n > 0) ? s2 / s1 : 0.0;
+  const double* ms = molscal + mp.index * MS_COUNT;
+  double phib = ms[MS_PHIB];
+  double lnpref;
+  if (mp.freedom == SFX_RESTRICTED)
+    lnpref = log(mp.theta) - ms[MS_LNGN] - (mp.kind == 0 ? 0.0 : log((double)mp.N));
+  else
+    lnpref = mp.kind == 0 ? log(phib) : log(phib / mp.N);
+  d[3] = (mp.kind == 2 ? 2.0 : 1.0) * exp(lnpref + ms[MS_REF]);
+  d[0] = dln;
+  if (mp.freedom == SFX_RESTRICTED) {
+    d[1] = -dln;
+    d[2] = -phib * dln;
+  } else if (mp.freedom == SFX_SOLVENT) {
+    double dpb = 0;
+    for (int m = 0; m < nmol; m++)
+      if (m != mp.index) dpb -= dmol[m * 4 + 2];
+    d[2] = dpb;
+    d[1] = phib != 0 ? dpb / phib : 0.0;
+  } else {
+    d[1] = 0;
+    d[2] = 0;
+  }
+}
+
+// dphi_{m,t} at interior sites
+template <typename T>
+__global__ void k_tmol_phi(DevLattice L, const double* __restrict__ phi_m, const T* __restrict__ dacc,
+                           const double* __restrict__ g, const double* __restrict__ v, const double* __restrict__ scal,
+                           const double* __restrict__ dm, int monomer, double* __restrict__ dphi_m) {
+  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= L.V) return;
+  int x, y, z;
+  interior_coords(L, q, x, y, z);
+  long long r = sidx(L, x, y, z);
+  double ph = phi_m[r];
+  double rel = dm[1];
+  double gv = g[r];
+  double out;
+  if (monomer)
+    out = ph * (rel - v[r]);
+  else
+    out = gv > 0 ? ph * (rel + v[r]) + dm[3] * (double)dacc[r] / gv : 0.0;
+  dphi_m[r] = out;
+}
+
+// tangent totals (frozen states have no tangent); halos from their sources
+__global__ void k_tphi_total(DevLattice L, int nstates, int nmol, const double* __restrict__ dphi_m,
+                             const int* __restrict__ mol_uses, const int* __restrict__ frozen,
+                             double* __restrict__ dphi, double* __restrict__ dphitot) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.M) return;
+  int x, y, z;
+  site_coords(L, i, x, y, z);
+  int sx = remap(x, L.nx, L.b1[0], L.b2[0]);
+  int sy = L.grad >= 2 ? remap(y, L.ny, L.b1[1], L.b2[1]) : y;
+  int sz = L.grad >= 3 ? remap(z, L.nz, L.b1[2], L.b2[2]) : z;
+  long long s = sidx(L, sx, sy, sz);
+  bool surf_halo = !is_interior(L, sx, sy, sz);
+  double tot = 0;
+  for (int t = 0; t < nstates; t++) {
+    double v = 0;
+    if (!frozen[t] && !surf_halo)
+      for (int m = 0; m < nmol; m++)
+        if (mol_uses[m * nstates + t]) v += dphi_m[((long long)m * nstates + t) * L.M + s];
+    dphi[(long long)t * L.M + i] = v;
+    tot += v;
+  }
+  dphitot[i] = tot;
+}
+
+__global__ void k_tsides(DevLattice L, int nstates, const double* __restrict__ dphi, double* __restrict__ drho) {
+  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= L.V) return;
+  int x, y, z;
+  interior_coords(L, q, x, y, z);
+  long long r = sidx(L, x, y, z);
+  for (int t = 0; t < nstates; t++) drho[(long long)t * L.M + r] = side_at(L, dphi + (long long)t * L.M, x, y, z);
+}
+
+// linearised residual (SF_Solve.cpp:554-590)
+__global__ void k_jres(DevLattice L, SysParams sp, const unsigned char* __restrict__ itmask,
+                       const double* __restrict__ v, const double* __restrict__ rho,
+                       const double* __restrict__ drho, const double* __restrict__ phitot,
+                       const double* __restrict__ dphitot, const double* __restrict__ molscal,
+                       const double* __restrict__ dmol, double* __restrict__ jv) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.M) return;
+  const int nv = sp.n_itvar;
+  if (!itmask[i]) {
+    for (int k = 0; k < nv; k++) jv[(long long)k * L.M + i] = 0.0;
+    return;
+  }
+  double pt = phitot[i], dpt = dphitot[i];
+  double du[SFX_MAXS];
+  double av = 0;
+  for (int k = 0; k < nv; k++) {
+    int t = sp.itvar_rep[k];
+    double ci = 0, dci = 0, dcb = 0;
+    for (int j = 0; j < sp.ns; j++) {
+      double c = sp.chi[t * SFX_MAXS + j];
+      if (c == 0) continue;
+      ci += c * rho[(long long)j * L.M + i];
+      dci += c * drho[(long long)j * L.M + i];
+      if (!sp.melt && !sp.frozen[j]) {
+        double dpb = 0;
+        for (int m = 0; m < sp.nm; m++) dpb += dmol[m * 4 + 2] * sp.mol_frac[m * SFX_MAXS + j];
+        dcb += c * dpb;
+      }
+    }
+    double d = v[(long long)k * L.M + i] - dci / pt + ci * dpt / (pt * pt) + dcb;
+    du[k] = d;
+    av += d;
+  }
+  av /= nv;
+  for (int k = 0; k < nv; k++) jv[(long long)k * L.M + i] = -dpt / (pt * pt) + du[k] - av;
+}
+
+// per-state direction v_t = v_{itvar(t)} (zero for frozen states), in T
+template <typename T>
+__global__ void k_state_dir(long long M, int ns, const int* st_itvar, const double* v, T* out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  for (int t = 0; t < ns; t++) {
+    int iv = st_itvar[t];
+    out[(long long)t * M + i] = iv >= 0 ? (T)v[(long long)iv * M + i] : T(0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, bool SCALED, bool CONNECT>
+static void tprop_k(const DevLattice& L, const T* Gin, const T* dGin, const T* g, const T* v, T* Gout, T* dGout,
+                    const T* Gp, const T* dGp, T* dacc, const RankScale& sc, hipStream_t st) {
+  unsigned nb = grid1(L.V, kTB);
+#define TP(K) k_tprop<T, K, SCALED, CONNECT><<<nb, kTB, 0, st>>>(L, Gin, dGin, g, v, Gout, dGout, Gp, dGp, dacc, sc)
+  switch (L.kind) {
+    case ST_1D: TP(ST_1D); break;
+    case ST_2D: TP(ST_2D); break;
+    case ST_3D_SC: TP(ST_3D_SC); break;
+    case ST_3D_FCC: TP(ST_3D_FCC); break;
+    case ST_3D_HEX: TP(ST_3D_HEX); break;
+  }
+#undef TP
+}
+template <typename T>
+static void tprop(const DevLattice& L, const T* Gin, const T* dGin, const T* g, const T* v, T* Gout, T* dGout,
+                  const T* Gp, const T* dGp, T* dacc, const RankScale& sc, hipStream_t st) {
+  bool scaled = sc.maxv != nullptr;
+  if (dacc) {
+    if (scaled) tprop_k<T, true, true>(L, Gin, dGin, g, v, Gout, dGout, Gp, dGp, dacc, sc, st);
+    else tprop_k<T, false, true>(L, Gin, dGin, g, v, Gout, dGout, Gp, dGp, dacc, sc, st);
+  } else {
+    if (scaled) tprop_k<T, true, false>(L, Gin, dGin, g, v, Gout, dGout, Gp, dGp, dacc, sc, st);
+    else tprop_k<T, false, false>(L, Gin, dGin, g, v, Gout, dGout, Gp, dGp, dacc, sc, st);
+  }
+}
+
+static __global__ void k_set_ref2(double* scal, const double* lnv, int idx, double factor) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) scal[MS_REF] = lnv ? factor * lnv[idx] : 0.0;
+}
+
+// Tangent sweep of molecule m with per-state directions vt (type T, ns*M).
+// The base propagators are recomputed alongside in the same precision; the
+// base phi_m comes from the last compute_phi at the same x.
+template <typename T>
+static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const T* g32or64, const double* v64) {
+  const DevLattice& L = S.lat.d;
+  const long long Mn = L.M;
+  Molecule& mo = S.mols[m];
+  hipStream_t st = S.stream;
+  double* scal = S.d_molscal + (size_t)m * MS_COUNT;
+  double* lnf = S.d_lnf + (size_t)m * (S.maxN + 2);
+  double* lnb = S.d_lnb + (size_t)m * (S.maxN + 2);
+  unsigned* maxf = S.d_maxf + (size_t)m * (S.maxN + 2);
+  unsigned* maxb = S.d_maxb + (size_t)m * (S.maxN + 2);
+  T* store = (T*)S.d_store;
+  T* dstore = (T*)W.dstore.p;
+  T* pp[2] = {(T*)S.d_pp, (T*)S.d_pp + Mn};
+  T* dpp[2] = {(T*)W.dpp.p, (T*)W.dpp.p + Mn};
+  T* dacc = (T*)W.dacc.p;
+  auto daccp = [&](int t) { return dacc + ((size_t)m * S.ns + t) * Mn; };
+  auto gp = [&](int t) { return g32or64 + (size_t)t * Mn; };
+  auto vp = [&](int t) { return vt + (size_t)t * Mn; };
+  auto rank = [&](int s) { return store + (size_t)(s - 1) * Mn; };
+  auto drank = [&](int s) { return dstore + (size_t)(s - 1) * Mn; };
+  const int N = mo.N;
+  for (int t = 0; t < S.ns; t++)
+    if (mo.uses[t]) zero<T>(daccp(t), Mn, st);
+  RankScale none{nullptr, nullptr, nullptr, nullptr, 0, 0};
+  if (scaled) {
+    zero<unsigned>(maxf, S.maxN + 2, st);
+    zero<unsigned>(maxb, S.maxN + 2, st);
+  }
+  auto fsc = [&](int step, int pair) { return scaled ? RankScale{maxf, lnf, lnf, scal + MS_REF, step, pair} : none; };
+  auto bsc = [&](int step, int pair) { return scaled ? RankScale{maxb, lnb, lnf, scal + MS_REF, step, pair} : none; };
+  const T* fin = nullptr;
+  const T* dfin = nullptr;
+  double* dm = W.dmol.p;
+  if (mo.kind == 0) {
+    int t = mo.seq[0];
+    unsigned nb = grid1(L.V, kTB);
+    k_wsum_mono<<<nb, kTB, 0, st>>>(L, S.d_swf + (size_t)t * Mn, v64 + (size_t)t * Mn, S.d_partial, W.partial2.p);
+    k_tmol_finalize<<<1, 1, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, (int)nb, S.d_molscal, dm, S.nm);
+  } else {
+    k_tinit<T><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), rank(1), drank(1), fsc(1, 0));
+    if (mo.kind == 1) {
+      for (int s = 2; s <= N; s++)
+        tprop<T>(L, rank(s - 1), drank(s - 1), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), rank(s), drank(s), nullptr,
+                 nullptr, nullptr, fsc(s, 0), st);
+      k_set_ref2<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
+      int tN = mo.seq[N - 1];
+      k_tinit<T><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(tN), vp(tN), pp[0], dpp[0], bsc(1, N));
+      if (scaled)
+        k_tconnect<T, true><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[0], dpp[0], rank(N), drank(N), daccp(tN), bsc(1, N), 1.0);
+      else
+        k_tconnect<T, false><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[0], dpp[0], rank(N), drank(N), daccp(tN), bsc(1, N), 1.0);
+      for (int s = N - 1; s >= 1; s--) {
+        int k = N - s + 1;
+        int t = mo.seq[s - 1];
+        tprop<T>(L, pp[(k - 2) & 1], dpp[(k - 2) & 1], gp(t), vp(t), pp[(k - 1) & 1], dpp[(k - 1) & 1], rank(s),
+                 drank(s), daccp(t), bsc(k, s), st);
+      }
+      fin = pp[(N - 1) & 1];
+      dfin = dpp[(N - 1) & 1];
+    } else {
+      const int h = N / 2;
+      for (int s = 2; s <= h; s++)
+        tprop<T>(L, rank(s - 1), drank(s - 1), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), rank(s), drank(s), nullptr,
+                 nullptr, nullptr, fsc(s, 0), st);
+      k_set_ref2<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, h, 2.0);
+      launch_copy_interior<T>(L, rank(h), pp[0], st);
+      launch_copy_interior<T>(L, drank(h), dpp[0], st);
+      int cur = 0;
+      if (N % 2 == 1) {
+        int s = h + 1, t = mo.seq[s - 1];
+        tprop<T>(L, pp[cur], dpp[cur], gp(t), vp(t), pp[cur ^ 1], dpp[cur ^ 1], nullptr, nullptr, nullptr, fsc(s, 0), st);
+        cur ^= 1;
+        RankScale c = fsc(s, s);
+        if (scaled)
+          k_tconnect<T, true><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[cur], dpp[cur], pp[cur], dpp[cur], daccp(t), c, 0.5);
+        else
+          k_tconnect<T, false><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[cur], dpp[cur], pp[cur], dpp[cur], daccp(t), c, 0.5);
+      }
+      for (int s = (N + 3) / 2; s <= N; s++) {
+        int sinv = N - s + 1, t = mo.seq[s - 1];
+        tprop<T>(L, pp[cur], dpp[cur], gp(t), vp(t), pp[cur ^ 1], dpp[cur ^ 1], rank(sinv), drank(sinv), daccp(t),
+                 fsc(s, sinv), st);
+        cur ^= 1;
+      }
+      fin = pp[cur];
+      dfin = dpp[cur];
+    }
+    unsigned nb = grid1(L.V, kTB);
+    k_wsum2<T><<<nb, kTB, 0, st>>>(L, fin, dfin, S.d_partial, W.partial2.p);
+    k_tmol_finalize<<<1, 1, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, (int)nb, S.d_molscal, dm, S.nm);
+  }
+}
+
+template <typename T>
+static void tmol_phi(System& S, Tangent& W, int m, const double* v64state) {
+  const DevLattice& L = S.lat.d;
+  const long long Mn = L.M;
+  Molecule& mo = S.mols[m];
+  for (int t = 0; t < S.ns; t++) {
+    if (!mo.uses[t]) continue;
+    size_t off = ((size_t)m * S.ns + t) * Mn;
+    k_tmol_phi<T><<<grid1(L.V, kTB), kTB, 0, S.stream>>>(
+        L, S.d_phi_m + off, (const T*)W.dacc.p + off, S.d_swf + (size_t)t * Mn, v64state + (size_t)t * Mn,
+        S.d_molscal + (size_t)m * MS_COUNT, W.dmol.p + m * 4, mo.kind == 0, W.dphi_m.p + off);
+  }
+}
+
+void tangent_free(Tangent* t) { delete t; }
+
+// J v at the base state of the last compute_phi / residual at S.d_u.
+void jvp_at_base(System& S, const double* v, double* jv, int precision) {
+  if (!S.tan) {
+    S.tan = new Tangent();
+    S.tan->init(S);
+  }
+  Tangent& W = *S.tan;
+  const DevLattice& L = S.lat.d;
+  const long long Mn = L.M;
+  hipStream_t st = S.stream;
+  double* v64 = W.v64.p;
+  k_state_dir<double><<<grid1(Mn, kTB), kTB, 0, st>>>(Mn, S.ns, S.d_st_itvar, v, v64);
+  bool fp32 = precision == SFX_FP32;
+  float* v32 = (float*)W.v32.p;
+  if (fp32) k_state_dir<float><<<grid1(Mn, kTB), kTB, 0, st>>>(Mn, S.ns, S.d_st_itvar, v, v32);
+  for (int pass = 0; pass < 2; pass++)
+    for (int m = 0; m < S.nm; m++) {
+      bool solv = S.mols[m].freedom == SFX_SOLVENT;
+      if ((pass == 0) == solv) continue;
+      if (fp32) {
+        tsweep<float>(S, W, m, true, v32, S.d_swf32, v64);
+        tmol_phi<float>(S, W, m, v64);
+      } else {
+        tsweep<double>(S, W, m, false, v64, S.d_swf, v64);
+        tmol_phi<double>(S, W, m, v64);
+      }
+    }
+  k_tphi_total<<<grid1(Mn, kTB), kTB, 0, st>>>(L, S.ns, S.nm, W.dphi_m.p, S.d_mol_uses, S.d_frozen, W.dphi.p,
+                                               W.dphitot.p);
+  k_tsides<<<grid1(L.V, kTB), kTB, 0, st>>>(L, S.ns, W.dphi.p, W.drho.p);
+  k_jres<<<grid1(Mn, kTB), kTB, 0, st>>>(L, S.sp, S.d_itmask, v, S.d_rho, W.drho.p, S.d_phitot, W.dphitot.p,
+                                         S.d_molscal, W.dmol.p, jv);
+  SFX_HIP(hipGetLastError());
+}
+
+void jvp_internal(System& S, const double* v, double* jv, int precision) {
+  S.compute_phi(precision);
+  jvp_at_base(S, v, jv, precision);
+}
+
+void vjp_internal(System& S, const double* w, double* jtw, int precision) {
+  (void)S;
+  (void)w;
+  (void)jtw;
+  (void)precision;
+  throw Error{SFX_E_UNSUPPORTED, "transpose products are not built yet"};
+}
+
+// ABI-layout vector (host or device) -> internal [itvar][M] (zero off sites)
+void load_vec(System& S, const double* x, int where, double* dst) {
+  // reuse load_x through a temporary swap of d_u
+  double* save = S.d_u;
+  S.d_u = dst;
+  try {
+    S.load_x(x, where);
+  } catch (...) {
+    S.d_u = save;
+    throw;
+  }
+  S.d_u = save;
+}
+
+}  // namespace sfx

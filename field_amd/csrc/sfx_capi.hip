@@ -1,0 +1,230 @@
+// sfx_capi.hip -- the extern "C" boundary (include/sfx.h). No exception
+// crosses it: every entry point converts failures into a status code and a
+// thread-local message.
+#include <cstdarg>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "sfx_solve.h"
+
+namespace sfx {
+
+static thread_local std::string g_msg;
+static thread_local int g_code = 0;
+
+void set_error(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_msg = buf;
+  g_code = code;
+}
+int last_code() { return g_code; }
+
+
+// implemented in sfx_adjoint.hip
+void jvp_internal(System& S, const double* v_internal, double* jv_internal, int precision);
+void vjp_internal(System& S, const double* w_internal, double* jtw_internal, int precision);
+void load_vec(System& S, const double* x, int where, double* dst_internal);
+
+}  // namespace sfx
+
+using namespace sfx;
+
+struct sfx_system {
+  System s;
+};
+
+extern "C" {
+
+int sfx_abi_version(void) { return SFX_ABI_VERSION; }
+
+int sfx_last_error(char* buf, size_t size) {
+  if (!buf || size == 0) return SFX_E_INVALID;
+  std::strncpy(buf, g_msg.c_str(), size - 1);
+  buf[size - 1] = 0;
+  return SFX_OK;
+}
+
+int sfx_set_device(int device) {
+  return guarded([&] { SFX_HIP(hipSetDevice(device)); });
+}
+
+int sfx_device_count(int* count) {
+  return guarded([&] {
+    SFX_REQUIRE(count, SFX_E_INVALID, "count is NULL");
+    SFX_HIP(hipGetDeviceCount(count));
+  });
+}
+
+int sfx_system_create(const sfx_lattice_desc* lat, const sfx_system_desc* sys, const sfx_options* opt,
+                      sfx_system** out) {
+  return guarded([&] {
+    SFX_REQUIRE(lat && sys && out, SFX_E_INVALID, "NULL descriptor");
+    *out = nullptr;
+    auto* h = new sfx_system();
+    try {
+      h->s.create(*lat, *sys, opt);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int sfx_system_destroy(sfx_system* s) {
+  return guarded([&] { delete s; });
+}
+
+int sfx_system_sizes(const sfx_system* s, long* nvar, long* M) {
+  return guarded([&] {
+    SFX_REQUIRE(s, SFX_E_INVALID, "NULL system");
+    if (nvar) *nvar = (long)s->s.nvar;
+    if (M) *M = (long)s->s.lat.d.M;
+  });
+}
+
+int sfx_compute_phi(sfx_system* s, const double* x, int where, int precision) {
+  return guarded([&] {
+    SFX_REQUIRE(s && x, SFX_E_INVALID, "NULL argument");
+    s->s.load_x(x, where);
+    s->s.compute_phi(precision);
+    s->s.sync();
+  });
+}
+
+int sfx_residual(sfx_system* s, const double* x, double* f, int where, int precision) {
+  return guarded([&] {
+    SFX_REQUIRE(s && x && f, SFX_E_INVALID, "NULL argument");
+    s->s.load_x(x, where);
+    s->s.residual_internal(precision);
+    s->s.store_f(s->s.d_f, f, where);
+  });
+}
+
+int sfx_jvp(sfx_system* s, const double* x, const double* v, double* jv, int where, int precision) {
+  return guarded([&] {
+    SFX_REQUIRE(s && x && v && jv, SFX_E_INVALID, "NULL argument");
+    System& S = s->s;
+    S.load_x(x, where);
+    DevBuf<double> vin((size_t)S.n_itvar * S.lat.d.M), out((size_t)S.n_itvar * S.lat.d.M);
+    load_vec(S, v, where, vin.p);
+    jvp_internal(S, vin.p, out.p, precision);
+    S.store_f(out.p, jv, where);
+  });
+}
+
+int sfx_vjp(sfx_system* s, const double* x, const double* w, double* jtw, int where, int precision) {
+  return guarded([&] {
+    SFX_REQUIRE(s && x && w && jtw, SFX_E_INVALID, "NULL argument");
+    System& S = s->s;
+    S.load_x(x, where);
+    DevBuf<double> win((size_t)S.n_itvar * S.lat.d.M), out((size_t)S.n_itvar * S.lat.d.M);
+    load_vec(S, w, where, win.p);
+    vjp_internal(S, win.p, out.p, precision);
+    S.store_f(out.p, jtw, where);
+  });
+}
+
+int sfx_get_phi(sfx_system* s, int state, double* phi_out, int where) {
+  return guarded([&] {
+    SFX_REQUIRE(s && phi_out, SFX_E_INVALID, "NULL argument");
+    System& S = s->s;
+    SFX_REQUIRE(state >= 0 && state < S.ns, SFX_E_INVALID, "state out of range");
+    SFX_REQUIRE(S.phi_valid, SFX_E_INVALID, "no phi computed yet");
+    SFX_HIP(hipMemcpyAsync(phi_out, S.d_phi + (size_t)state * S.lat.d.M, S.lat.d.M * sizeof(double),
+                           where == SFX_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, S.stream));
+    S.sync();
+  });
+}
+
+static int get_molscal(sfx_system* s, int molecule, int which, double* out) {
+  return guarded([&] {
+    SFX_REQUIRE(s && out, SFX_E_INVALID, "NULL argument");
+    System& S = s->s;
+    SFX_REQUIRE(molecule >= 0 && molecule < S.nm, SFX_E_INVALID, "molecule out of range");
+    SFX_REQUIRE(S.phi_valid, SFX_E_INVALID, "no phi computed yet");
+    SFX_HIP(hipMemcpyAsync(out, S.d_molscal + (size_t)molecule * MS_COUNT + which, sizeof(double),
+                           hipMemcpyDeviceToHost, S.stream));
+    S.sync();
+  });
+}
+
+int sfx_get_lngn(sfx_system* s, int molecule, double* lngn) { return get_molscal(s, molecule, MS_LNGN, lngn); }
+int sfx_get_phibulk(sfx_system* s, int molecule, double* phibulk) {
+  return get_molscal(s, molecule, MS_PHIB, phibulk);
+}
+
+int sfx_free_energy(sfx_system* s, double* F) {
+  return guarded([&] {
+    SFX_REQUIRE(s && F, SFX_E_INVALID, "NULL argument");
+    *F = s->s.free_energy();
+  });
+}
+
+int sfx_solver_defaults(sfx_solver_opts* o) {
+  if (!o) return SFX_E_INVALID;
+  std::memset(o, 0, sizeof *o);
+  o->method = SFX_SOLVER_DIIS;
+  o->tolerance = 1e-7;    // SF_Solve.cpp:1836, MPS.cpp:13
+  o->max_iter = 1000;     // MPS.cpp:273
+  o->diis_m = 10;         // MPS.cpp:306
+  o->diis_eta = 0.1;      // MPS.cpp:13
+  o->gmres_restart = 30;
+  o->gmres_max_iter = 200;
+  o->gmres_rtol = 1e-3;
+  o->jvp_precision = SFX_FP32;
+  o->fp64_switch = 1e-3;
+  o->verbose = 0;
+  return SFX_OK;
+}
+
+int sfx_solve(sfx_system* s, const sfx_solver_opts* o, double* x, int where, sfx_solve_stats* stats) {
+  return guarded([&] {
+    SFX_REQUIRE(s && o && x, SFX_E_INVALID, "NULL argument");
+    System& S = s->s;
+    sfx_solve_stats st{};
+    S.load_x(x, where);
+    if (o->method == SFX_SOLVER_DIIS)
+      solve_diis(S, *o, st);
+    else if (o->method == SFX_SOLVER_NK)
+      solve_nk(S, *o, st);
+    else
+      throw Error{SFX_E_INVALID, "unknown solver method"};
+    S.store_f(S.d_u, x, where);
+    if (stats) *stats = st;
+    if (!st.converged) throw Error{SFX_E_NOTCONVERGED, "solver reached its iteration limit"};
+  });
+}
+
+// ---- per-rank lattice primitives --------------------------------------------
+int sfx_lattice_create(const sfx_lattice_desc* d, sfx_lattice** out) {
+  return guarded([&] {
+    SFX_REQUIRE(d && out, SFX_E_INVALID, "NULL argument");
+    auto* h = new sfx_lattice();
+    try {
+      lattice_init(h->l, *d);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int sfx_lattice_destroy(sfx_lattice* l) {
+  return guarded([&] { delete l; });
+}
+
+int sfx_lattice_sites(const sfx_lattice* l, long* M) {
+  return guarded([&] {
+    SFX_REQUIRE(l && M, SFX_E_INVALID, "NULL argument");
+    *M = (long)l->l.d.M;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// sfx_kernels.h -- launch wrappers and kernel parameter blocks.
+#pragma once
+
+#include "sfx_internal.h"
+
+#define SFX_MAXS 8  // segment states per system
+#define SFX_MAXM 8  // molecules per system
+
+namespace sfx {
+
+// per-molecule device scalars
+enum { MS_LNGN = 0, MS_LNC, MS_PHIB, MS_PREF, MS_REF, MS_COUNT = 8 };
+
+// FP32 per-rank log-scaling context of one sweep. Step k writes max|G| of its
+// output into maxv[k] and the cumulative log scale into lnv[k]; its output is
+// multiplied by 1/maxv[k-1]. Composition multiplies by
+// exp(lnv[k] + lnpair[pair] - *ref). maxv == nullptr disables scaling.
+struct RankScale {
+  unsigned* maxv;
+  double* lnv;
+  const double* lnpair;
+  const double* ref;
+  int step;
+  int pair;
+};
+
+struct MolParams {
+  int index, N, freedom, kind;  // kind: 0 monomer, 1 full sweep, 2 half-stored (palindrome)
+  double phib, theta;
+};
+
+struct SysParams {
+  int ns, nm, n_itvar, melt;
+  double chi[SFX_MAXS * SFX_MAXS];
+  int frozen[SFX_MAXS];
+  int st_itvar[SFX_MAXS];
+  int itvar_rep[SFX_MAXS];
+  double melt_phib[SFX_MAXS];
+  double mol_frac[SFX_MAXM * SFX_MAXS];
+  double mol_N[SFX_MAXM];
+  double mol_ref[SFX_MAXM];
+};
+
+template <typename T, typename A>
+void launch_prop(const DevLattice& L, const T* Gin, const T* g, T* Gout, const T* Gpair, A* acc,
+                 const RankScale& sc, hipStream_t st);
+template <typename T>
+void launch_init_rank(const DevLattice& L, const T* g, T* out, const RankScale& sc, hipStream_t st);
+template <typename T, typename A>
+void launch_connect(const DevLattice& L, const T* a, const T* b, A* acc, const RankScale& sc, double half,
+                    hipStream_t st);
+template <typename T>
+void launch_copy_interior(const DevLattice& L, const T* a, T* b, hipStream_t st);
+template <typename T>
+int launch_wsum(const DevLattice& L, const T* G, double* partial, hipStream_t st);
+void launch_mol_finalize(const MolParams& mp, const double* partial, int nparts, double* scal,
+                         const double* lnfN, const double* allscal, int nmol, hipStream_t st);
+template <typename A>
+void launch_mol_phi(const DevLattice& L, const A* acc, const double* g, const double* scal, int monomer,
+                    double* phi_m, hipStream_t st);
+void launch_swf(const DevLattice& L, const double* u, const unsigned char* itmask, int nstates,
+                const int* st_itvar, double* swf, float* swf32, hipStream_t st);
+void launch_phi_total(const DevLattice& L, int nstates, int nmol, const double* phi_m, const int* mol_uses,
+                      const double* fprof, const int* frozen, double* phi, double* phitot, hipStream_t st);
+void launch_sides(const DevLattice& L, int nstates, const double* phi, double* rho, hipStream_t st);
+void launch_bulk(const SysParams& sp, const double* molscal, double* phib_state, double* cbulk, hipStream_t st);
+void launch_residual(const DevLattice& L, const SysParams& sp, const unsigned char* itmask, const double* swf,
+                     const double* rho, const double* phitot, const double* cbulk, double* f, hipStream_t st);
+int launch_free_energy(const DevLattice& L, const SysParams& sp, const double* phi, const double* phi_m,
+                       const double* rho, const double* swf, const double* u, const double* molscal,
+                       double* partial, hipStream_t st);
+void launch_sum_partials(const double* partial, int n, double* out, hipStream_t st);
+
+}  // namespace sfx

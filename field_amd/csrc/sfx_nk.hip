@@ -1,0 +1,190 @@
+// sfx_nk.hip -- matrix-free Newton-Krylov on the device.
+//
+// Newton step J(x) dx = -f(x) solved by restarted GMRES whose products J v are
+// the exact tangent-propagator JVPs (sfx_adjoint.hip), run in FP32 with
+// per-rank log-scaling. The residual f is evaluated with FP32 sweeps while
+// ||f|| is above fp64_switch and with FP64 sweeps below it; convergence is
+// declared on the FP64 residual only, so the FP32 Jacobian only steers the
+// step (iterative refinement). Backtracking on ||f|| globalises the step.
+#include <cmath>
+
+#include "sfx_solve.h"
+
+namespace sfx {
+
+void jvp_at_base(System& S, const double* v, double* jv, int precision);
+
+void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
+  const long long n = (long long)S.n_itvar * S.lat.d.M;
+  const int m = std::max(1, std::min(o.gmres_restart, 30));
+  VecOps V;
+  V.init(n, S.stream);
+  DevBuf<double> basis((size_t)(m + 1) * n), w(n), dx(n), x0(n), r(n);
+  double* x = S.d_u;
+  auto Vk = [&](int k) { return basis.p + (size_t)k * n; };
+  hipEvent_t e0, e1;
+  SFX_HIP(hipEventCreate(&e0));
+  SFX_HIP(hipEventCreate(&e1));
+  SFX_HIP(hipEventRecord(e0, S.stream));
+
+  int prec = o.fp64_switch > 0 ? SFX_FP32 : SFX_FP64;
+  auto eval = [&](int p) {
+    S.residual_internal(p);
+    if (p == SFX_FP32)
+      st.residual_evals_fp32++;
+    else
+      st.residual_evals++;
+    V.copy(S.d_f, r.p);
+    return std::sqrt(V.dot(r.p, r.p));
+  };
+  double rn = eval(prec);
+  int it = 0;
+  bool converged = false;
+  while (true) {
+    if (!std::isfinite(rn)) throw Error{SFX_E_NONFINITE, "non-finite residual in Newton-Krylov"};
+    if (prec == SFX_FP32 && rn < o.fp64_switch) {
+      prec = SFX_FP64;
+      rn = eval(prec);
+    }
+    if (o.verbose) std::fprintf(stderr, "nk it=%d |f|=%.6e prec=%s\n", it, rn, prec == SFX_FP64 ? "fp64" : "fp32");
+    if (prec == SFX_FP64 && rn < o.tolerance) {
+      converged = true;
+      break;
+    }
+    if (it >= o.max_iter) break;
+    it++;
+    // ---- GMRES(m) for J dx = -r, dx0 = 0 ----
+    double eta = o.gmres_rtol;
+    V.axpby(0.0, r.p, 0.0, dx.p);  // dx = 0
+    int total = 0;
+    double beta0 = rn;
+    for (int cycle = 0; cycle < 1 + o.gmres_max_iter / m && total < o.gmres_max_iter; cycle++) {
+      // residual of the linear system: r_l = -r - J dx
+      double beta;
+      if (cycle == 0) {
+        V.axpby(-1.0 / rn, r.p, 0.0, Vk(0));
+        beta = rn;
+      } else {
+        jvp_at_base(S, dx.p, w.p, o.jvp_precision);
+        st.jvp_evals++;
+        V.axpby(-1.0, r.p, -1.0, w.p);  // w = -r - J dx
+        beta = std::sqrt(V.dot(w.p, w.p));
+        if (beta <= eta * beta0) break;
+        V.axpby(1.0 / beta, w.p, 0.0, Vk(0));
+      }
+      std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0);
+      g[0] = beta;
+      int j = 0;
+      double res = beta;
+      for (; j < m && total < o.gmres_max_iter; j++) {
+        jvp_at_base(S, Vk(j), w.p, o.jvp_precision);
+        st.jvp_evals++;
+        total++;
+        // classical Gram-Schmidt, twice
+        std::vector<double> h(j + 1, 0.0);
+        for (int pass = 0; pass < 2; pass++) {
+          DotList dl{};
+          std::vector<double> hh(j + 1);
+          for (int base = 0; base <= j; base += 16) {
+            int cnt = std::min(16, j + 1 - base);
+            dl.count = cnt;
+            for (int i = 0; i < cnt; i++) {
+              dl.a[i] = Vk(base + i);
+              dl.b[i] = w.p;
+            }
+            V.dots(dl, hh.data() + base);
+          }
+          VecList vl{};
+          std::vector<double> c(j + 2);
+          vl.count = j + 2;
+          for (int i = 0; i <= j; i++) {
+            vl.v[i] = Vk(i);
+            c[i] = -hh[i];
+            h[i] += hh[i];
+          }
+          vl.v[j + 1] = w.p;
+          c[j + 1] = 1.0;
+          V.combine(vl, c.data(), w.p);
+        }
+        double hn = std::sqrt(V.dot(w.p, w.p));
+        for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = h[i];
+        H[(size_t)(j + 1) * m + j] = hn;
+        // apply previous rotations, then a new one
+        for (int i = 0; i < j; i++) {
+          double a = H[(size_t)i * m + j], b = H[(size_t)(i + 1) * m + j];
+          H[(size_t)i * m + j] = cs[i] * a + sn[i] * b;
+          H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+        }
+        double a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
+        double d = std::hypot(a, b);
+        cs[j] = d > 0 ? a / d : 1.0;
+        sn[j] = d > 0 ? b / d : 0.0;
+        H[(size_t)j * m + j] = d;
+        H[(size_t)(j + 1) * m + j] = 0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        res = std::fabs(g[j + 1]);
+        if (hn > 0) V.axpby(1.0 / hn, w.p, 0.0, Vk(j + 1));
+        if (res <= eta * beta0 || hn == 0) {
+          j++;
+          break;
+        }
+      }
+      // y = H^-1 g, dx += V y
+      int k = j;
+      std::vector<double> y(k, 0.0);
+      for (int i = k - 1; i >= 0; i--) {
+        double s = g[i];
+        for (int l = i + 1; l < k; l++) s -= H[(size_t)i * m + l] * y[l];
+        y[i] = s / H[(size_t)i * m + i];
+      }
+      VecList vl{};
+      std::vector<double> c(k + 1);
+      vl.count = k + 1;
+      for (int i = 0; i < k; i++) {
+        vl.v[i] = Vk(i);
+        c[i] = y[i];
+      }
+      vl.v[k] = dx.p;
+      c[k] = 1.0;
+      V.combine(vl, c.data(), dx.p);
+      if (o.verbose) std::fprintf(stderr, "  gmres cycle=%d its=%d est=%.3e\n", cycle, j, res / beta0);
+      if (res <= eta * beta0) break;
+    }
+    // ---- backtracking line search on ||f|| ----
+    V.copy(x, x0.p);
+    double alpha = 1.0, rt = 0;
+    bool accepted = false;
+    for (int ls = 0; ls < 12; ls++) {
+      V.copy(x0.p, x);
+      V.axpby(alpha, dx.p, 1.0, x);
+      rt = eval(prec);
+      if (std::isfinite(rt) && rt <= (1 - 1e-4 * alpha) * rn) {
+        accepted = true;
+        break;
+      }
+      alpha *= 0.5;
+    }
+    if (!accepted) {
+      // keep the smallest step; the next Newton iteration re-linearises there
+      if (o.verbose) std::fprintf(stderr, "  line search failed, |f|=%.3e\n", rt);
+    }
+    rn = rt;
+  }
+  SFX_HIP(hipEventRecord(e1, S.stream));
+  SFX_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  SFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  st.iterations = it;
+  st.residual_norm = rn;
+  st.seconds = ms * 1e-3;
+  st.converged = converged;
+  if (prec != SFX_FP64) {  // report the FP64 residual of the returned point
+    S.residual_internal(SFX_FP64);
+    st.residual_norm = std::sqrt(V.dot(S.d_f, S.d_f));
+  }
+}
+
+}  // namespace sfx

@@ -1,0 +1,247 @@
+// sfx_solve.hip -- device-resident SCF solvers.
+//
+// DIIS follows MPS.cpp:228-284 (memory m, step eta, sliding window, the
+// coefficients solve A c = 1 then normalise) with every vector on the device:
+// per iteration one residual evaluation, one multi-dot kernel, one host solve
+// of an m x m system, one combination kernel. Vectors use the internal
+// [itvar][M] layout with zeros off the iteration sites, so their dot products
+// equal those of the reference's iteration-variable vectors.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "sfx_solve.h"
+
+namespace sfx {
+
+constexpr int kDotBlock = 256;
+constexpr int kMaxDots = 16;
+
+// partial[d][block] = sum_i a_d[i] * b_d[i]
+static __global__ void __launch_bounds__(kDotBlock) k_multidot(DotList dl, long long n, double* partial) {
+  __shared__ double sm[kMaxDots][kDotBlock / 64];
+  double acc[kMaxDots];
+  for (int d = 0; d < dl.count; d++) acc[d] = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    for (int d = 0; d < dl.count; d++) acc[d] += dl.a[d][i] * dl.b[d][i];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int d = 0; d < dl.count; d++) {
+    double v = acc[d];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) sm[d][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < dl.count) {
+    double s = 0;
+    for (int k = 0; k < kDotBlock / 64; k++) s += sm[threadIdx.x][k];
+    partial[(long long)threadIdx.x * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+static __global__ void k_reduce_rows(const double* partial, int nrows, int ncols, double* out) {
+  int r = threadIdx.x;
+  if (r >= nrows) return;
+  double s = 0;
+  for (int c = 0; c < ncols; c++) s += partial[(long long)r * ncols + c];
+  out[r] = s;
+}
+
+// y = sum_k c_k v_k  (c in device memory)
+static __global__ void k_combine(VecList vl, const double* c, long long n, double* y) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0;
+  for (int k = 0; k < vl.count; k++) s += c[k] * vl.v[k][i];
+  y[i] = s;
+}
+
+// y = a*x + b*y
+static __global__ void k_axpby(double a, const double* x, double b, double* y, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+// z = x - y
+static __global__ void k_sub(const double* x, const double* y, double* z, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = x[i] - y[i];
+}
+static __global__ void k_copy(const double* x, double* y, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+
+void VecOps::init(long long n_, hipStream_t st) {
+  n = n_;
+  stream = st;
+  nblk = (int)std::min<long long>(grid1(n, kDotBlock), 1024);
+  SFX_HIP(hipMalloc(&partial, sizeof(double) * kMaxDots * nblk));
+  SFX_HIP(hipMalloc(&dres, sizeof(double) * kMaxDots));
+  SFX_HIP(hipMalloc(&dcoef, sizeof(double) * kMaxDots));
+}
+VecOps::~VecOps() {
+  if (partial) (void)hipFree(partial);
+  if (dres) (void)hipFree(dres);
+  if (dcoef) (void)hipFree(dcoef);
+}
+void VecOps::dots(const DotList& dl, double* host_out) {
+  SFX_REQUIRE(dl.count <= kMaxDots, SFX_E_INVALID, "too many dot products");
+  k_multidot<<<nblk, kDotBlock, 0, stream>>>(dl, n, partial);
+  k_reduce_rows<<<1, 64, 0, stream>>>(partial, dl.count, nblk, dres);
+  SFX_HIP(hipMemcpyAsync(host_out, dres, sizeof(double) * dl.count, hipMemcpyDeviceToHost, stream));
+  SFX_HIP(hipStreamSynchronize(stream));
+}
+double VecOps::dot(const double* a, const double* b) {
+  DotList dl{};
+  dl.a[0] = a;
+  dl.b[0] = b;
+  dl.count = 1;
+  double r;
+  dots(dl, &r);
+  return r;
+}
+void VecOps::combine(const VecList& vl, const double* host_c, double* y) {
+  SFX_HIP(hipMemcpyAsync(dcoef, host_c, sizeof(double) * vl.count, hipMemcpyHostToDevice, stream));
+  k_combine<<<grid1(n, 256), 256, 0, stream>>>(vl, dcoef, n, y);
+}
+void VecOps::axpby(double a, const double* x, double b, double* y) {
+  k_axpby<<<grid1(n, 256), 256, 0, stream>>>(a, x, b, y, n);
+}
+void VecOps::sub(const double* x, const double* y, double* z) { k_sub<<<grid1(n, 256), 256, 0, stream>>>(x, y, z, n); }
+void VecOps::copy(const double* x, double* y) { k_copy<<<grid1(n, 256), 256, 0, stream>>>(x, y, n); }
+
+// X = A^{-1} 1 for the symmetric DIIS matrix: Jacobi eigen-decomposition,
+// X = Q diag(1/w) Q^T 1 (the pseudo-inverse of MPS.cpp:200-226 for a
+// non-singular A).
+void diis_coeffs(const std::vector<double>& Ain, int n, std::vector<double>& X) {
+  std::vector<double> A(Ain), Q(n * n, 0.0), w(n);
+  for (int i = 0; i < n; i++) Q[i * n + i] = 1;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0;
+    for (int i = 0; i < n; i++)
+      for (int j = i + 1; j < n; j++) off += A[i * n + j] * A[i * n + j];
+    if (off < 1e-300) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        double apq = A[p * n + q];
+        if (apq == 0) continue;
+        double th = (A[q * n + q] - A[p * n + p]) / (2 * apq);
+        double t = (th >= 0 ? 1 : -1) / (std::fabs(th) + std::sqrt(th * th + 1));
+        double c = 1 / std::sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < n; k++) {
+          double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * akq;
+          A[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * aqk;
+          A[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          double qkp = Q[k * n + p], qkq = Q[k * n + q];
+          Q[k * n + p] = c * qkp - s * qkq;
+          Q[k * n + q] = s * qkp + c * qkq;
+        }
+      }
+  }
+  for (int i = 0; i < n; i++) w[i] = A[i * n + i];
+  X.assign(n, 0.0);
+  for (int k = 0; k < n; k++) {
+    double qt1 = 0;
+    for (int i = 0; i < n; i++) qt1 += Q[i * n + k];
+    double c = qt1 / w[k];
+    for (int i = 0; i < n; i++) X[i] += Q[i * n + k] * c;
+  }
+}
+
+void solve_diis(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
+  const long long n = (long long)S.n_itvar * S.lat.d.M;
+  const int m = o.diis_m;
+  const double eta = o.diis_eta;
+  SFX_REQUIRE(m >= 1 && m <= 15, SFX_E_INVALID, "diis_m must be in 1..15");
+  VecOps V;
+  V.init(n, S.stream);
+  DevBuf<double> x0(n), xR((size_t)m * n), x_x0((size_t)m * n);
+  double* x = S.d_u;
+  double* g = S.d_f;
+  auto vec = [&](DevBuf<double>& b, int k) { return b.p + (size_t)k * n; };
+  hipEvent_t e0, e1;
+  SFX_HIP(hipEventCreate(&e0));
+  SFX_HIP(hipEventCreate(&e1));
+  SFX_HIP(hipEventRecord(e0, S.stream));
+  std::vector<double> Aij(m * m, 0.0), Apij, Ci;
+  int it = 0, k_diis = 1, k = 0;
+  V.copy(x, x0.p);  // remember the starting point for x_x0 = x - x0 (x0 = 0 in MPS.cpp:264)
+  S.residual_internal(SFX_FP64);
+  st.residual_evals++;
+  V.axpby(-eta, g, 1.0, x);
+  V.sub(x, x0.p, vec(x_x0, 0));
+  V.copy(x, vec(xR, 0));
+  double err = std::sqrt(V.dot(g, g));
+  while (err > o.tolerance && it < o.max_iter) {
+    it++;
+    V.copy(x, x0.p);
+    S.residual_internal(SFX_FP64);
+    st.residual_evals++;
+    k = it % m;
+    k_diis++;
+    V.axpby(-eta, g, 1.0, x);
+    V.copy(x, vec(xR, k));
+    V.sub(x, x0.p, vec(x_x0, k));
+    if (k_diis > m) {
+      k_diis = m;
+      for (int i = 1; i < m; i++)
+        for (int j = 1; j < m; j++) Aij[m * (i - 1) + j - 1] = Aij[m * i + j];
+    }
+    DotList dl{};
+    dl.count = k_diis + 1;
+    for (int i = 0; i < k_diis; i++) {
+      int posi = k - k_diis + 1 + i;
+      if (posi < 0) posi += m;
+      dl.a[i] = vec(x_x0, posi);
+      dl.b[i] = vec(x_x0, k);
+    }
+    dl.a[k_diis] = g;
+    dl.b[k_diis] = g;
+    double d[kMaxDots];
+    V.dots(dl, d);
+    for (int i = 0; i < k_diis; i++) Aij[i + m * (k_diis - 1)] = Aij[k_diis - 1 + m * i] = d[i];
+    err = std::sqrt(d[k_diis]);
+    if (err <= o.tolerance) {
+      V.copy(x0.p, x);  // return the point whose residual was measured
+      break;
+    }
+    Apij.assign(k_diis * k_diis, 0.0);
+    for (int i = 0; i < k_diis; i++)
+      for (int j = 0; j < k_diis; j++) Apij[j + k_diis * i] = Aij[j + m * i];
+    diis_coeffs(Apij, k_diis, Ci);
+    double normC = 0;
+    for (int i = 0; i < k_diis; i++) normC += Ci[i];
+    for (int i = 0; i < k_diis; i++) Ci[i] /= normC;
+    VecList vl{};
+    vl.count = k_diis;
+    for (int i = 0; i < k_diis; i++) {
+      int posi = k - k_diis + 1 + i;
+      if (posi < 0) posi += m;
+      vl.v[i] = vec(xR, posi);
+    }
+    V.combine(vl, Ci.data(), x);
+    if (!std::isfinite(err)) throw Error{SFX_E_NONFINITE, "non-finite residual in DIIS"};
+    if (o.verbose) std::fprintf(stderr, "diis it=%d |f|=%.6e\n", it, err);
+  }
+  SFX_HIP(hipEventRecord(e1, S.stream));
+  SFX_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  SFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // leave phi / free energy consistent with x
+  S.residual_internal(SFX_FP64);
+  S.sync();
+  st.iterations = it;
+  st.residual_norm = err;
+  st.seconds = ms * 1e-3;
+  st.converged = err <= o.tolerance;
+}
+
+}  // namespace sfx

@@ -1,0 +1,88 @@
+// sfx_system.h -- one SCF system on one GPU (host side).
+#pragma once
+
+#include <memory>
+
+#include "sfx_kernels.h"
+
+namespace sfx {
+
+struct Molecule {
+  int N = 1, freedom = SFX_FREE, kind = 1;  // kind: 0 monomer, 1 full sweep, 2 half-stored
+  std::vector<int> seq;
+  std::vector<int> uses;  // per state: 1 if the chain contains it
+  double phib = 0, theta = 0, chem_int_ref = 0;
+  int nstore = 0;         // ranks kept in the store
+  MolParams params(int index) const {
+    return MolParams{index, N, freedom, kind, phib, theta};
+  }
+};
+
+struct Tangent;  // JVP / VJP buffers (sfx_adjoint.hip)
+void tangent_free(Tangent* t);
+
+struct System {
+  Lattice lat;
+  int ns = 0, nm = 0, form = SFX_FORM_SFBOX, n_itvar = 0, precision = SFX_FP64;
+  size_t budget = 0;
+  std::vector<double> chi, fprof, melt_phib;
+  std::vector<int> frozen, st_itvar, itvar_rep;
+  std::vector<Molecule> mols;
+  std::vector<long long> sites;  // iteration sites (SF_Solve layout order)
+  std::vector<unsigned char> itmask;
+  long long nvar = 0;
+  int maxN = 1, maxstore = 1;
+  SysParams sp{};
+  hipStream_t stream = nullptr;
+
+  // device buffers
+  double* d_u = nullptr;      // n_itvar * M
+  double* d_f = nullptr;      // n_itvar * M (internal residual layout)
+  double* d_swf = nullptr;    // ns * M
+  float* d_swf32 = nullptr;   // ns * M
+  double* d_phi_m = nullptr;  // nm * ns * M
+  void* d_acc = nullptr;      // nm * ns * M accumulators (double or float)
+  double* d_phi = nullptr;    // ns * M
+  double* d_rho = nullptr;    // ns * M
+  double* d_phitot = nullptr; // M
+  double* d_fprof = nullptr;  // ns * M
+  int* d_frozen = nullptr;
+  int* d_st_itvar = nullptr;
+  int* d_mol_uses = nullptr;  // nm * ns
+  unsigned char* d_itmask = nullptr;
+  long long* d_sites = nullptr;
+  void* d_store = nullptr;    // maxstore * M propagator words
+  void* d_pp = nullptr;       // 2 * M running propagator
+  double* d_molscal = nullptr;  // nm * MS_COUNT
+  double* d_lnf = nullptr;      // nm * (maxN+2)
+  double* d_lnb = nullptr;
+  unsigned* d_maxf = nullptr;
+  unsigned* d_maxb = nullptr;
+  double* d_partial = nullptr;  // reduction partials
+  double* d_phib_state = nullptr;
+  double* d_cbulk = nullptr;
+  double* d_scalar = nullptr;   // misc device scalars
+  double* d_xabi = nullptr;     // nvar: ABI-layout staging
+  int npart = 0;
+  Tangent* tan = nullptr;
+  bool phi_valid = false;
+  int last_precision = SFX_FP64;
+
+  ~System();
+  void create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const sfx_options* opt);
+  long long M() const { return lat.d.M; }
+
+  // x (ABI layout, host or device) -> d_u
+  void load_x(const double* x, int where);
+  // d_f -> f (ABI layout, host or device)
+  void store_f(const double* src_internal, double* f, int where);
+  void compute_phi(int precision);  // from d_u
+  void residual_internal(int precision);  // d_u -> d_f
+  double free_energy();
+  void sync();
+
+  template <typename T, typename A>
+  void sweep(int m, bool scaled);
+};
+
+}  // namespace sfx

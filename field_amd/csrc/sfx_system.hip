@@ -1,0 +1,436 @@
+// sfx_system.hip -- system set-up and the SCF function evaluation on the GPU.
+//
+// The evaluation follows the reference's call chain SF_Solve::residuals ->
+// UpdateSWF -> SF_MolList1stO::ComputePhi -> per-molecule Matrix1/SymMatrix1/
+// monomer -> GetResiduals (SF_Solve.cpp:647-656), with every array on the
+// device and every step a kernel on one stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "sfx_system.h"
+
+namespace sfx {
+
+static __global__ void k_fill_u32(unsigned* p, unsigned v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+static __global__ void k_fill_bytes8(unsigned long long* p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+// scal[MS_REF] = factor * lnv[idx] (lnv == nullptr: 0)
+static __global__ void k_set_ref(double* scal, const double* lnv, int idx, double factor) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) scal[MS_REF] = lnv ? factor * lnv[idx] : 0.0;
+}
+// x (SF_Solve layout) -> u[itvar][M]; u zero off the iteration sites
+static __global__ void k_unpack_sfbox(const double* x, const long long* sites, long long nsites, int nv,
+                                      long long M, double* u) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsites) return;
+  for (int i = 0; i < nv; i++) u[(long long)i * M + sites[k]] = x[k * nv + i];
+}
+static __global__ void k_pack_sfbox(const double* u, const long long* sites, long long nsites, int nv,
+                                    long long M, double* x) {
+  long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsites) return;
+  for (int i = 0; i < nv; i++) x[k * nv + i] = u[(long long)i * M + sites[k]];
+}
+static __global__ void k_mask_copy(const double* x, const unsigned char* mask, int nv, long long M, double* u) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  bool m = mask[i];
+  for (int k = 0; k < nv; k++) u[(long long)k * M + i] = m ? x[(long long)k * M + i] : 0.0;
+}
+
+static void zero_bytes(void* p, size_t bytes, hipStream_t st) {
+  long long n = (long long)((bytes + 7) / 8);
+  if (n) k_fill_bytes8<<<grid1(n, 256), 256, 0, st>>>((unsigned long long*)p, n);
+}
+
+template <typename T>
+static T* dalloc(size_t n) {
+  T* p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) throw Error{SFX_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e)};
+  return p;
+}
+
+System::~System() {
+  void* ptrs[] = {d_u, d_f, d_swf, d_swf32, d_phi_m, d_acc, d_phi, d_rho, d_phitot, d_fprof, d_frozen,
+                  d_st_itvar, d_mol_uses, d_itmask, d_sites, d_store, d_pp, d_molscal, d_lnf, d_lnb,
+                  d_maxf, d_maxb, d_partial, d_phib_state, d_cbulk, d_scalar, d_xabi};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  tangent_free(tan);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+static int is_palindrome(const std::vector<int>& s) {
+  for (size_t i = 0; i < s.size() / 2; i++)
+    if (s[i] != s[s.size() - 1 - i]) return 0;
+  return 1;
+}
+
+void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const sfx_options* opt) {
+  lattice_init(lat, ld);
+  SFX_REQUIRE(ld.order != 2, SFX_E_UNSUPPORTED, "second-order (semiflexible) systems use sfx_system2_create");
+  const long long Mn = lat.d.M;
+  ns = sd.n_states;
+  nm = sd.n_molecules;
+  form = sd.form;
+  SFX_REQUIRE(ns >= 1 && ns <= SFX_MAXS, SFX_E_INVALID, "n_states must be in 1..8");
+  SFX_REQUIRE(nm >= 1 && nm <= SFX_MAXM, SFX_E_INVALID, "n_molecules must be in 1..8");
+  SFX_REQUIRE(sd.chi && sd.molecules, SFX_E_INVALID, "chi and molecules are required");
+  precision = opt ? opt->precision : SFX_FP64;
+  budget = opt ? opt->memory_budget : 0;
+  chi.assign(sd.chi, sd.chi + ns * ns);
+  for (int i = 0; i < ns; i++)
+    for (int j = 0; j < ns; j++)
+      SFX_REQUIRE(chi[i * ns + j] == chi[j * ns + i], SFX_E_INVALID, "chi must be symmetric");
+  frozen.assign(ns, 0);
+  if (sd.frozen) frozen.assign(sd.frozen, sd.frozen + ns);
+  fprof.assign((size_t)ns * Mn, 0.0);
+  bool any_frozen = false;
+  for (int t = 0; t < ns; t++) any_frozen |= frozen[t] != 0;
+  if (any_frozen) {
+    SFX_REQUIRE(sd.frozen_profile != nullptr, SFX_E_INVALID, "frozen states need frozen_profile");
+    for (int t = 0; t < ns; t++)
+      if (frozen[t]) std::copy(sd.frozen_profile + (size_t)t * Mn, sd.frozen_profile + (size_t)(t + 1) * Mn,
+                               fprof.begin() + (size_t)t * Mn);
+  }
+  melt_phib.assign(ns, 0.0);
+  if (form == SFX_FORM_MELT && sd.melt_phibulk) melt_phib.assign(sd.melt_phibulk, sd.melt_phibulk + ns);
+  // iteration variables: liquid states with equal chi rows share one (SF_Solve.cpp:53-86)
+  st_itvar.assign(ns, -1);
+  itvar_rep.clear();
+  n_itvar = 0;
+  for (int i = 0; i < ns; i++) {
+    if (frozen[i]) continue;
+    for (int j = 0; j < i && st_itvar[i] < 0; j++) {
+      if (frozen[j]) continue;
+      bool eq = true;
+      for (int k = 0; k < ns; k++) eq &= chi[i * ns + k] == chi[j * ns + k];
+      if (eq) st_itvar[i] = st_itvar[j];
+    }
+    if (st_itvar[i] < 0) {
+      st_itvar[i] = n_itvar++;
+      itvar_rep.push_back(i);
+    } else {
+      itvar_rep[st_itvar[i]] = i;  // the last state of a group (SF_Solve.cpp:563-564)
+    }
+  }
+  SFX_REQUIRE(n_itvar >= 1, SFX_E_INVALID, "no liquid states");
+  // iteration sites (SF_Solve.cpp:135-150)
+  itmask.assign(Mn, 0);
+  sites.clear();
+  for (long long i = 0; i < Mn; i++) {
+    int z = (int)(i % lat.d.MZ);
+    long long r = i / lat.d.MZ;
+    int y = (int)(r % lat.d.MY), x = (int)(r / lat.d.MY);
+    bool ok = x >= 1 && x <= lat.d.nx && (lat.d.grad < 2 || (y >= 1 && y <= lat.d.ny)) &&
+              (lat.d.grad < 3 || (z >= 1 && z <= lat.d.nz));
+    for (int t = 0; ok && t < ns; t++)
+      if (frozen[t] && fprof[(size_t)t * Mn + i] > 0.999) ok = false;
+    itmask[i] = ok;
+    if (ok) sites.push_back(i);
+  }
+  nvar = form == SFX_FORM_MELT ? (long long)n_itvar * Mn : (long long)sites.size() * n_itvar;
+  // molecules
+  mols.assign(nm, Molecule{});
+  int nsolv = 0;
+  maxN = 1;
+  maxstore = 1;
+  for (int m = 0; m < nm; m++) {
+    const sfx_molecule_desc& d = sd.molecules[m];
+    Molecule& mo = mols[m];
+    SFX_REQUIRE(d.length >= 1 && d.sequence, SFX_E_INVALID, "molecule needs a sequence");
+    mo.N = d.length;
+    mo.seq.assign(d.sequence, d.sequence + d.length);
+    mo.freedom = d.freedom;
+    mo.phib = d.phibulk;
+    mo.theta = d.theta;
+    mo.uses.assign(ns, 0);
+    for (int t : mo.seq) {
+      SFX_REQUIRE(t >= 0 && t < ns && !frozen[t], SFX_E_INVALID, "sequence names a frozen or unknown state");
+      mo.uses[t] = 1;
+    }
+    if (d.freedom == SFX_SOLVENT) nsolv++;
+    if (d.freedom == SFX_RESTRICTED) SFX_REQUIRE(d.theta > 0, SFX_E_INVALID, "restricted molecule needs theta > 0");
+    if (d.freedom == SFX_FREE) SFX_REQUIRE(d.phibulk > 0, SFX_E_INVALID, "free molecule needs phibulk > 0");
+    bool homo = std::all_of(mo.seq.begin(), mo.seq.end(), [&](int t) { return t == mo.seq[0]; });
+    mo.kind = mo.N == 1 ? 0 : ((homo || is_palindrome(mo.seq)) ? 2 : 1);
+    mo.nstore = mo.kind == 0 ? 0 : mo.kind == 2 ? mo.N / 2 : mo.N;
+    maxN = std::max(maxN, mo.N);
+    maxstore = std::max(maxstore, mo.nstore);
+    // chemIntRef (SF_MolStructure.cpp:412-424)
+    double ref = 0;
+    for (int a = 0; a < ns; a++) {
+      double fa = std::count(mo.seq.begin(), mo.seq.end(), a) / (double)mo.N;
+      if (fa == 0) continue;
+      double ci = 0;
+      for (int b = 0; b < ns; b++)
+        ci += chi[a * ns + b] * std::count(mo.seq.begin(), mo.seq.end(), b) / (double)mo.N;
+      ref += fa * ci;
+    }
+    mo.chem_int_ref = ref / 2;
+  }
+  if (form == SFX_FORM_SFBOX)
+    SFX_REQUIRE(nsolv == 1, SFX_E_INVALID, "exactly one solvent molecule is required (SF_MoleculeList.cpp:63-82)");
+  else
+    SFX_REQUIRE(nsolv == 0, SFX_E_INVALID, "a melt has no solvent");
+  // parameter block
+  sp = SysParams{};
+  sp.ns = ns;
+  sp.nm = nm;
+  sp.n_itvar = n_itvar;
+  sp.melt = form == SFX_FORM_MELT;
+  for (int i = 0; i < ns; i++) {
+    for (int j = 0; j < ns; j++) sp.chi[i * SFX_MAXS + j] = chi[i * ns + j];
+    sp.frozen[i] = frozen[i];
+    sp.st_itvar[i] = st_itvar[i];
+    sp.melt_phib[i] = melt_phib[i];
+  }
+  for (int k = 0; k < n_itvar; k++) sp.itvar_rep[k] = itvar_rep[k];
+  for (int m = 0; m < nm; m++) {
+    sp.mol_N[m] = mols[m].N;
+    sp.mol_ref[m] = mols[m].chem_int_ref;
+    for (int t = 0; t < ns; t++)
+      sp.mol_frac[m * SFX_MAXS + t] =
+          std::count(mols[m].seq.begin(), mols[m].seq.end(), t) / (double)mols[m].N;
+  }
+  // device buffers
+  SFX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  const size_t M = (size_t)Mn;
+  d_u = dalloc<double>(n_itvar * M);
+  d_f = dalloc<double>(n_itvar * M);
+  d_swf = dalloc<double>(ns * M);
+  d_swf32 = dalloc<float>(ns * M);
+  d_phi_m = dalloc<double>((size_t)nm * ns * M);
+  d_acc = dalloc<double>((size_t)nm * ns * M);
+  d_phi = dalloc<double>(ns * M);
+  d_rho = dalloc<double>(ns * M);
+  d_phitot = dalloc<double>(M);
+  d_fprof = dalloc<double>(ns * M);
+  d_frozen = dalloc<int>(ns);
+  d_st_itvar = dalloc<int>(ns);
+  d_mol_uses = dalloc<int>((size_t)nm * ns);
+  d_itmask = dalloc<unsigned char>(M);
+  d_sites = dalloc<long long>(sites.size());
+  size_t word = precision == SFX_FP32 ? 4 : 8;
+  SFX_REQUIRE(budget == 0 || (size_t)maxstore * M * 8 <= budget, SFX_E_UNSUPPORTED,
+              "memory budget below the full propagator store: checkpointed sweeps are not built for this "
+              "system type");
+  d_store = dalloc<double>((size_t)maxstore * M);  // sized for FP64 so both precisions fit
+  (void)word;
+  d_pp = dalloc<double>(2 * M);
+  d_molscal = dalloc<double>((size_t)nm * MS_COUNT);
+  d_lnf = dalloc<double>((size_t)nm * (maxN + 2));
+  d_lnb = dalloc<double>((size_t)nm * (maxN + 2));
+  d_maxf = dalloc<unsigned>((size_t)nm * (maxN + 2));
+  d_maxb = dalloc<unsigned>((size_t)nm * (maxN + 2));
+  npart = (int)grid1(std::max<long long>(lat.d.M, lat.d.V), 256);
+  d_partial = dalloc<double>(npart);
+  d_phib_state = dalloc<double>(SFX_MAXS);
+  d_cbulk = dalloc<double>(SFX_MAXS);
+  d_scalar = dalloc<double>(16);
+  d_xabi = dalloc<double>((size_t)nvar);
+  std::vector<int> uses((size_t)nm * ns);
+  for (int m = 0; m < nm; m++)
+    for (int t = 0; t < ns; t++) uses[(size_t)m * ns + t] = mols[m].uses[t];
+  SFX_HIP(hipMemcpy(d_fprof, fprof.data(), ns * M * sizeof(double), hipMemcpyHostToDevice));
+  SFX_HIP(hipMemcpy(d_frozen, frozen.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+  SFX_HIP(hipMemcpy(d_st_itvar, st_itvar.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+  SFX_HIP(hipMemcpy(d_mol_uses, uses.data(), uses.size() * sizeof(int), hipMemcpyHostToDevice));
+  SFX_HIP(hipMemcpy(d_itmask, itmask.data(), M, hipMemcpyHostToDevice));
+  if (!sites.empty())
+    SFX_HIP(hipMemcpy(d_sites, sites.data(), sites.size() * sizeof(long long), hipMemcpyHostToDevice));
+  // zero everything once: halo slots of propagator arrays are never written
+  zero_bytes(d_store, (size_t)maxstore * M * 8, stream);
+  zero_bytes(d_pp, 2 * M * 8, stream);
+  zero_bytes(d_u, n_itvar * M * 8, stream);
+  zero_bytes(d_f, n_itvar * M * 8, stream);
+  zero_bytes(d_acc, (size_t)nm * ns * M * 8, stream);
+  zero_bytes(d_phi_m, (size_t)nm * ns * M * 8, stream);
+  zero_bytes(d_molscal, (size_t)nm * MS_COUNT * 8, stream);
+  SFX_HIP(hipStreamSynchronize(stream));
+  SFX_HIP(hipGetLastError());
+}
+
+void System::sync() { SFX_HIP(hipStreamSynchronize(stream)); }
+
+void System::load_x(const double* x, int where) {
+  const long long Mn = lat.d.M;
+  const double* src = x;
+  if (where == SFX_HOST) {
+    SFX_HIP(hipMemcpyAsync(d_xabi, x, nvar * sizeof(double), hipMemcpyHostToDevice, stream));
+    src = d_xabi;
+  }
+  if (form == SFX_FORM_MELT) {
+    k_mask_copy<<<grid1(Mn, 256), 256, 0, stream>>>(src, d_itmask, n_itvar, Mn, d_u);
+  } else {
+    zero_bytes(d_u, (size_t)n_itvar * Mn * 8, stream);
+    k_unpack_sfbox<<<grid1((long long)sites.size(), 256), 256, 0, stream>>>(src, d_sites, (long long)sites.size(),
+                                                                            n_itvar, Mn, d_u);
+  }
+  SFX_HIP(hipGetLastError());
+}
+
+// internal [itvar][M] vector -> ABI layout (host or device)
+void System::store_f(const double* src, double* f, int where) {
+  const long long Mn = lat.d.M;
+  const hipMemcpyKind kind = where == SFX_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  if (form == SFX_FORM_MELT) {
+    SFX_HIP(hipMemcpyAsync(f, src, nvar * sizeof(double), kind, stream));
+  } else {
+    double* dst = where == SFX_HOST ? d_xabi : f;
+    k_pack_sfbox<<<grid1((long long)sites.size(), 256), 256, 0, stream>>>(src, d_sites, (long long)sites.size(),
+                                                                          n_itvar, Mn, dst);
+    if (where == SFX_HOST) SFX_HIP(hipMemcpyAsync(f, dst, nvar * sizeof(double), kind, stream));
+  }
+  SFX_HIP(hipStreamSynchronize(stream));
+}
+
+// One molecule: forward sweep into the store, backward sweep fused with the
+// composition, ln G_N and the normalisation constants
+// (SF_Copol1stO.cpp:1186-1281, SF_Copol1stO.cpp:1630-1729, SF_Homopol1stO.cpp:552-622).
+template <typename T, typename A>
+void System::sweep(int m, bool scaled) {
+  const DevLattice& L = lat.d;
+  const long long Mn = L.M;
+  Molecule& mo = mols[m];
+  hipStream_t st = stream;
+  double* scal = d_molscal + (size_t)m * MS_COUNT;
+  double* lnf = d_lnf + (size_t)m * (maxN + 2);
+  double* lnb = d_lnb + (size_t)m * (maxN + 2);
+  unsigned* maxf = d_maxf + (size_t)m * (maxN + 2);
+  unsigned* maxb = d_maxb + (size_t)m * (maxN + 2);
+  T* store = (T*)d_store;
+  T* pp[2] = {(T*)d_pp, (T*)d_pp + Mn};
+  A* acc = (A*)d_acc;
+  auto accp = [&](int t) { return acc + ((size_t)m * ns + t) * Mn; };
+  auto gp = [&](int t) -> const T* {
+    if constexpr (sizeof(T) == 4) return (const T*)(d_swf32 + (size_t)t * Mn);
+    else return (const T*)(d_swf + (size_t)t * Mn);
+  };
+  auto rank = [&](int s) { return store + (size_t)(s - 1) * Mn; };
+  const int N = mo.N;
+  for (int t = 0; t < ns; t++)
+    if (mo.uses[t]) zero_bytes(accp(t), (size_t)Mn * sizeof(A), st);
+  if (scaled) {
+    k_fill_u32<<<grid1(N + 2, 64), 64, 0, st>>>(maxf, 0u, N + 2);
+    k_fill_u32<<<grid1(N + 2, 64), 64, 0, st>>>(maxb, 0u, N + 2);
+  }
+  RankScale none{nullptr, nullptr, nullptr, nullptr, 0, 0};
+  auto fsc = [&](int step, int pair) {
+    return scaled ? RankScale{maxf, lnf, lnf, scal + MS_REF, step, pair} : none;
+  };
+  auto bsc = [&](int step, int pair) {
+    return scaled ? RankScale{maxb, lnb, lnf, scal + MS_REF, step, pair} : none;
+  };
+  const T* final_G = nullptr;
+  const double* lnfinal = nullptr;
+  if (mo.kind == 0) {
+    int t = mo.seq[0];
+    int nb = launch_wsum<double>(L, d_swf + (size_t)t * Mn, d_partial, st);
+    k_set_ref<<<1, 1, 0, st>>>(scal, nullptr, 0, 0.0);
+    launch_mol_finalize(mo.params(m), d_partial, nb, scal, nullptr, d_molscal, nm, st);
+    launch_mol_phi<double>(L, nullptr, d_swf + (size_t)t * Mn, scal, 1, d_phi_m + ((size_t)m * ns + t) * Mn, st);
+    return;
+  }
+  launch_init_rank<T>(L, gp(mo.seq[0]), rank(1), fsc(1, 0), st);
+  if (mo.kind == 1) {
+    for (int s = 2; s <= N; s++)
+      launch_prop<T, A>(L, rank(s - 1), gp(mo.seq[s - 1]), rank(s), nullptr, nullptr, fsc(s, 0), st);
+    k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
+    // backward: step k = N-s+1 holds G_inv(s)
+    launch_init_rank<T>(L, gp(mo.seq[N - 1]), pp[0], bsc(1, N), st);
+    launch_connect<T, A>(L, pp[0], rank(N), accp(mo.seq[N - 1]), bsc(1, N), 1.0, st);
+    for (int s = N - 1; s >= 1; s--) {
+      int k = N - s + 1;
+      launch_prop<T, A>(L, pp[(k - 2) & 1], gp(mo.seq[s - 1]), pp[(k - 1) & 1], rank(s), accp(mo.seq[s - 1]),
+                        bsc(k, s), st);
+    }
+    final_G = pp[(N - 1) & 1];
+    lnfinal = scaled ? lnb + N : nullptr;
+  } else {
+    const int h = N / 2;
+    for (int s = 2; s <= h; s++)
+      launch_prop<T, A>(L, rank(s - 1), gp(mo.seq[s - 1]), rank(s), nullptr, nullptr, fsc(s, 0), st);
+    k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, h, 2.0);
+    // the running sweep continues the forward chain (ranks h+1..N)
+    launch_copy_interior<T>(L, rank(h), pp[0], st);
+    int cur = 0;
+    if (N % 2 == 1) {
+      int s = h + 1;
+      launch_prop<T, A>(L, pp[cur], gp(mo.seq[s - 1]), pp[cur ^ 1], nullptr, nullptr, fsc(s, 0), st);
+      cur ^= 1;
+      RankScale c = fsc(s, s);
+      launch_connect<T, A>(L, pp[cur], pp[cur], accp(mo.seq[s - 1]), c, 0.5, st);
+    }
+    for (int s = (N + 3) / 2; s <= N; s++) {
+      int sinv = N - s + 1;
+      launch_prop<T, A>(L, pp[cur], gp(mo.seq[s - 1]), pp[cur ^ 1], rank(sinv), accp(mo.seq[s - 1]), fsc(s, sinv),
+                        st);
+      cur ^= 1;
+    }
+    final_G = pp[cur];
+    lnfinal = scaled ? lnf + N : nullptr;
+  }
+  int nb = launch_wsum<T>(L, final_G, d_partial, st);
+  launch_mol_finalize(mo.params(m), d_partial, nb, scal, lnfinal, d_molscal, nm, st);
+  for (int t = 0; t < ns; t++)
+    if (mo.uses[t]) launch_mol_phi<A>(L, accp(t), d_swf + (size_t)t * Mn, scal, 0, d_phi_m + ((size_t)m * ns + t) * Mn, st);
+}
+
+void System::compute_phi(int prec) {
+  const DevLattice& L = lat.d;
+  launch_swf(L, d_u, d_itmask, ns, d_st_itvar, d_swf, d_swf32, stream);
+  // non-solvent molecules first, the solvent closes phi_b (SF_MolList1stO.cpp:280-292)
+  for (int pass = 0; pass < 2; pass++)
+    for (int m = 0; m < nm; m++) {
+      bool solv = mols[m].freedom == SFX_SOLVENT;
+      if ((pass == 0) == solv) continue;
+      if (prec == SFX_FP32)
+        sweep<float, float>(m, true);
+      else
+        sweep<double, double>(m, false);
+    }
+  launch_phi_total(L, ns, nm, d_phi_m, d_mol_uses, d_fprof, d_frozen, d_phi, d_phitot, stream);
+  launch_sides(L, ns, d_phi, d_rho, stream);
+  launch_bulk(sp, d_molscal, d_phib_state, d_cbulk, stream);
+  SFX_HIP(hipGetLastError());
+  phi_valid = true;
+  last_precision = prec;
+}
+
+void System::residual_internal(int prec) {
+  compute_phi(prec);
+  launch_residual(lat.d, sp, d_itmask, d_swf, d_rho, d_phitot, d_cbulk, d_f, stream);
+  SFX_HIP(hipGetLastError());
+}
+
+double System::free_energy() {
+  SFX_REQUIRE(phi_valid, SFX_E_INVALID, "free energy needs a preceding compute_phi or residual");
+  int nb = launch_free_energy(lat.d, sp, d_phi, d_phi_m, d_rho, d_swf, d_u, d_molscal, d_partial, stream);
+  launch_sum_partials(d_partial, nb, d_scalar, stream);
+  double F = 0;
+  std::vector<double> ms((size_t)nm * MS_COUNT);
+  SFX_HIP(hipMemcpyAsync(&F, d_scalar, sizeof(double), hipMemcpyDeviceToHost, stream));
+  SFX_HIP(hipMemcpyAsync(ms.data(), d_molscal, ms.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+  SFX_HIP(hipStreamSynchronize(stream));
+  if (form == SFX_FORM_MELT) {
+    for (int m = 0; m < nm; m++) {
+      double n = mols[m].theta / mols[m].N;
+      F += n * std::log(n * mols[m].N) - n * ms[(size_t)m * MS_COUNT + MS_LNGN];
+    }
+  }
+  return F;
+}
+
+template void System::sweep<double, double>(int, bool);
+template void System::sweep<float, float>(int, bool);
+
+}  // namespace sfx

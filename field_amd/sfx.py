@@ -1,0 +1,204 @@
+"""ctypes binding of libsfx.so (the C ABI in include/sfx.h).
+
+The product path: every call goes to the HIP library. If the library is not
+built this module raises at load time; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import _abi
+from .spec import LatticeSpec, SystemSpec
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsfx.so")
+_lib = None
+
+PD = C.POINTER(C.c_double)
+
+# every entry point of include/sfx.h with its ctypes signature
+_SIGS = {
+    "sfx_abi_version": (C.c_int, []),
+    "sfx_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "sfx_set_device": (C.c_int, [C.c_int]),
+    "sfx_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "sfx_system_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "sfx_system_destroy": (C.c_int, [C.c_void_p]),
+    "sfx_system_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    "sfx_compute_phi": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+    "sfx_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+    "sfx_jvp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+    "sfx_vjp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+    "sfx_get_phi": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    "sfx_get_lngn": (C.c_int, [C.c_void_p, C.c_int, PD]),
+    "sfx_get_phibulk": (C.c_int, [C.c_void_p, C.c_int, PD]),
+    "sfx_free_energy": (C.c_int, [C.c_void_p, PD]),
+    "sfx_solver_defaults": (C.c_int, [C.c_void_p]),
+    "sfx_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "sfx_lattice_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "sfx_lattice_destroy": (C.c_int, [C.c_void_p]),
+    "sfx_lattice_sites": (C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
+    "sfx_lattice_propagate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfx_lattice_connect": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfx_lattice_correct_double_count": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfx_lattice_lngn": (C.c_int, [C.c_void_p, C.c_void_p, PD, C.c_void_p]),
+    "sfx_lattice_set_boundaries": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfx_lattice_sides": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+}
+
+
+class SFError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sfx error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build the HIP library first (python -m field_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().sfx_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int):
+    if rc != 0:
+        raise SFError(rc, last_error())
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().sfx_device_count(C.byref(n)))
+    return n.value
+
+
+def _ptr(a):
+    """(pointer, where) for a numpy array or a device tensor."""
+    if isinstance(a, np.ndarray):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        return C.c_void_p(a.ctypes.data), _abi.HOST
+    # torch tensor on the GPU
+    assert a.dtype.__str__() == "torch.float64" and a.is_contiguous()
+    return C.c_void_p(a.data_ptr()), (_abi.DEVICE if a.is_cuda else _abi.HOST)
+
+
+_PREC = {"fp64": _abi.FP64, "fp32": _abi.FP32, None: None}
+
+
+class System:
+    """One SCF system on the GPU (sfx_system_create)."""
+
+    def __init__(self, lat: LatticeSpec, sysd: SystemSpec, precision: str = "fp64",
+                 memory_budget: int = 0):
+        self.lat, self.sys = lat, sysd
+        self._ld = lat.desc()
+        self._sd, self._keep = sysd.desc(lat)
+        opt = _abi.Options()
+        opt.precision = _PREC[precision]
+        opt.memory_budget = memory_budget
+        self.precision = opt.precision
+        h = C.c_void_p()
+        check(lib().sfx_system_create(C.byref(self._ld), C.byref(self._sd), C.byref(opt), C.byref(h)))
+        self.h = h
+        nv, M = C.c_long(), C.c_long()
+        check(lib().sfx_system_sizes(self.h, C.byref(nv), C.byref(M)))
+        self.nvar, self.M = nv.value, M.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sfx_system_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _p(self, precision):
+        return self.precision if precision is None else _PREC[precision]
+
+    def _vec(self, x):
+        if isinstance(x, np.ndarray):
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            assert x.size == self.nvar, (x.size, self.nvar)
+        return x
+
+    def residual(self, x, precision=None, out=None):
+        x = self._vec(x)
+        if out is None:
+            out = np.zeros(self.nvar) if isinstance(x, np.ndarray) else x.new_zeros(self.nvar)
+        px, where = _ptr(x)
+        po, _ = _ptr(out)
+        check(lib().sfx_residual(self.h, px, po, where, self._p(precision)))
+        return out
+
+    def compute_phi(self, x, precision=None):
+        x = self._vec(x)
+        px, where = _ptr(x)
+        check(lib().sfx_compute_phi(self.h, px, where, self._p(precision)))
+
+    def jvp(self, x, v, precision=None):
+        x, v = self._vec(x), self._vec(v)
+        out = np.zeros(self.nvar) if isinstance(x, np.ndarray) else x.new_zeros(self.nvar)
+        px, where = _ptr(x)
+        check(lib().sfx_jvp(self.h, px, _ptr(v)[0], _ptr(out)[0], where, self._p(precision)))
+        return out
+
+    def vjp(self, x, w, precision=None):
+        x, w = self._vec(x), self._vec(w)
+        out = np.zeros(self.nvar) if isinstance(x, np.ndarray) else x.new_zeros(self.nvar)
+        px, where = _ptr(x)
+        check(lib().sfx_vjp(self.h, px, _ptr(w)[0], _ptr(out)[0], where, self._p(precision)))
+        return out
+
+    def phi(self, state: int) -> np.ndarray:
+        out = np.zeros(self.M)
+        check(lib().sfx_get_phi(self.h, state, C.c_void_p(out.ctypes.data), _abi.HOST))
+        return out
+
+    def lngn(self, mol: int) -> float:
+        v = C.c_double()
+        check(lib().sfx_get_lngn(self.h, mol, C.byref(v)))
+        return v.value
+
+    def phibulk(self, mol: int) -> float:
+        v = C.c_double()
+        check(lib().sfx_get_phibulk(self.h, mol, C.byref(v)))
+        return v.value
+
+    def free_energy(self) -> float:
+        v = C.c_double()
+        check(lib().sfx_free_energy(self.h, C.byref(v)))
+        return v.value
+
+    def solve(self, x0, method="diis", check_converged=True, **kw):
+        o = _abi.SolverOpts()
+        check(lib().sfx_solver_defaults(C.byref(o)))
+        o.method = {"diis": _abi.SOLVER_DIIS, "nk": _abi.SOLVER_NK}[method]
+        for k, v in kw.items():
+            if k == "jvp_precision":
+                v = _PREC[v]
+            setattr(o, k, v)
+        x = self._vec(x0)
+        x = x.copy() if isinstance(x, np.ndarray) else x.clone()
+        st = _abi.SolveStats()
+        px, where = _ptr(x)
+        rc = lib().sfx_solve(self.h, C.byref(o), px, where, C.byref(st))
+        stats = {k: getattr(st, k) for k, _ in _abi.SolveStats._fields_}
+        if rc != 0 and not (rc == _abi.SFX_E_NOTCONVERGED and not check_converged):
+            raise SFError(rc, last_error())
+        return x, stats

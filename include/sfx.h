@@ -1,0 +1,216 @@
+/*
+ * sfx.h -- C ABI of the MI355X (gfx950) Scheutjens-Fleer hot path.
+ *
+ * Plain C, no HIP or torch types in any signature. Every entry point returns an
+ * int status (SFX_OK == 0, negative on error) and leaves a message for
+ * sfx_last_error(). Arrays are caller-owned; device buffers are owned by the
+ * handle that allocated them.
+ *
+ * Site layout (identical to the reference's lattice vectors, 0-based here):
+ *   a lattice with g gradients and n_layers[k] interior layers keeps one halo
+ *   layer on each side of every used gradient: MX = n_layers[0]+2,
+ *   MY = n_layers[1]+2 (2-D/3-D, else 1), MZ = n_layers[2]+2 (3-D, else 1),
+ *   M = MX*MY*MZ and site (x,y,z) sits at (x*MY + y)*MZ + z, so the last used
+ *   gradient is contiguous (Lat1DFlat.cpp:25, Lat2DFlat.cpp:44-46,
+ *   Lat3D.cpp:59-68). Interior coordinates run 1..n_layers[k].
+ *
+ * Reference interfaces replaced (file:line in HaripriyaGnanasekaran/field):
+ *   - Cuda_tools.h:3-26 (flat C GPU API): sfx_lattice_* primitives below.
+ *   - Lat1stO.h:19-50 / Lat2ndO.h:73-119 (virtual lattice API): the same
+ *     primitives, called per rank by the molecule drivers.
+ *   - SF_Copol1stO/SF_Homopol1stO/SF_Monomer1stO ::ComputePhi (Matrix1,
+ *     SymMatrix1, Matrix1Long) and SF_MolList1stO.cpp:246-295: sfx_compute_phi.
+ *   - SF_Solve::residuals / GetResiduals (SF_Solve.cpp:527-591, 647-656) and
+ *     MPS.cpp:187-198 ComputeG: sfx_residual.
+ *   - SF_MolList1stO::GetFreeEnergy (SF_MolList1stO.cpp:606-635) and
+ *     MPS.cpp:253-261 Helmholtz: sfx_free_energy.
+ *   - SFNewton::iterate / PlainLBFGSDIIS (sfnewton.cpp:566-664, 844-1143) and
+ *     MPS.cpp:228-284 DIIS/SCF: sfx_solve.
+ */
+#ifndef SFX_H
+#define SFX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFX_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  SFX_OK = 0,
+  SFX_E_INVALID = -1,      /* bad argument or descriptor */
+  SFX_E_OOM = -2,          /* device allocation failed */
+  SFX_E_UNSUPPORTED = -3,  /* combination not implemented */
+  SFX_E_NONFINITE = -4,    /* NaN/Inf produced (e.g. ln of a non-positive G_N) */
+  SFX_E_HIP = -5,          /* HIP runtime error */
+  SFX_E_NOTCONVERGED = -6  /* solver hit its iteration limit */
+};
+
+/* ---- lattice ----------------------------------------------------------- */
+enum { SFX_FLAT = 0, SFX_CYLINDER = 1, SFX_SPHERE = 2 };
+enum { SFX_LAT_STANDARD = 0, SFX_LAT_FCC = 1, SFX_LAT_HEX = 2 };
+/* Boundary choices, numbered as the reference's input choices
+ * (Lat1DFlat.cpp:26-31: surface, mirror1, mirror2, periodic). 3-D lattices
+ * accept mirror1/mirror2/periodic only (Lat3D.cpp:69-73). */
+enum {
+  SFX_BOUND_SURFACE = 1,
+  SFX_BOUND_MIRROR1 = 2,
+  SFX_BOUND_MIRROR2 = 3,
+  SFX_BOUND_PERIODIC = 4
+};
+
+typedef struct sfx_lattice_desc {
+  int gradients;          /* 1, 2 or 3 */
+  int geometry;           /* SFX_FLAT / SFX_CYLINDER / SFX_SPHERE (curved: 1-D, 2-D cyl) */
+  int lattice_type;       /* SFX_LAT_STANDARD / SFX_LAT_FCC / SFX_LAT_HEX (3-D) */
+  int order;              /* 1: flexible chains, 2: semiflexible (direction-resolved) */
+  int n_layers[3];        /* interior layers per gradient */
+  int lowerbound[3];      /* SFX_BOUND_* per gradient */
+  int upperbound[3];
+  double lambda;          /* 1-D/2-D step weight l1 (required, Lat1DFlat.cpp:109) */
+  double lambda_chi;      /* contact-average weight; <= 0 selects lambda */
+  double layer_adjustment;   /* 1-D: default 1 (Lat1DFlat.cpp:139) */
+  double offset_first_layer; /* curved geometries: default 0 */
+} sfx_lattice_desc;
+
+/* ---- system (one SCF problem on one lattice) ---------------------------- */
+enum { SFX_FREE = 0, SFX_RESTRICTED = 1, SFX_SOLVENT = 2 };
+
+typedef struct sfx_molecule_desc {
+  int length;             /* chain length N */
+  const int* sequence;    /* N state indices (0-based); rank s=1..N uses sequence[s-1] */
+  int freedom;            /* SFX_FREE (phibulk), SFX_RESTRICTED (theta), SFX_SOLVENT */
+  double phibulk;         /* SFX_FREE: bulk volume fraction */
+  double theta;           /* SFX_RESTRICTED: amount, sum over interior sites of phi */
+} sfx_molecule_desc;
+
+/* Residual / bookkeeping conventions:
+ *  SFX_FORM_SFBOX: SF_Solve::GetResiduals with solvent closure
+ *                  (SF_MolList1stO.cpp:280-292); x is the SF_Solve layout:
+ *                  for every iteration site (interior, not frozen) in site
+ *                  order, one value per distinct iteration variable.
+ *  SFX_FORM_MELT:  MPS.cpp incompressible melt: no solvent, explicit bulk
+ *                  fractions in the chi term (melt_phibulk), x is
+ *                  [state][M] with halo entries zero (MPS.cpp:309-320). */
+enum { SFX_FORM_SFBOX = 0, SFX_FORM_MELT = 1 };
+
+typedef struct sfx_system_desc {
+  int n_states;                   /* segment types, one state each */
+  const double* chi;              /* n_states*n_states, row-major, symmetric */
+  const int* frozen;              /* n_states flags (NULL: none frozen) */
+  const double* frozen_profile;   /* n_states*M range values of frozen states (rows of
+                                     liquid states ignored); NULL if none frozen */
+  int n_molecules;
+  const sfx_molecule_desc* molecules;
+  double stiffness;               /* 2nd order: straight-bond weight S; L=(1-S)/(2d-2) */
+  int form;                       /* SFX_FORM_SFBOX / SFX_FORM_MELT */
+  const double* melt_phibulk;     /* SFX_FORM_MELT: n_states bulk fractions */
+} sfx_system_desc;
+
+/* Propagator precision of a sweep. FP32 carries a per-rank log-scale. */
+enum { SFX_FP64 = 0, SFX_FP32 = 1 };
+
+typedef struct sfx_options {
+  int precision;          /* default precision of sfx_residual/sfx_compute_phi */
+  size_t memory_budget;   /* bytes for the stored propagator; 0 = unlimited.
+                             A smaller budget selects sqrt(N) checkpointing. */
+} sfx_options;
+
+typedef struct sfx_system sfx_system;
+
+/* Pointer-location flag for array arguments. */
+enum { SFX_HOST = 0, SFX_DEVICE = 1 };
+
+/* ---- library ----------------------------------------------------------- */
+int sfx_abi_version(void);
+/* Copies the last error message of the calling thread into buf. */
+int sfx_last_error(char* buf, size_t size);
+/* Selects the HIP device for handles created afterwards by this thread. */
+int sfx_set_device(int device);
+int sfx_device_count(int* count);
+
+/* ---- system lifecycle -------------------------------------------------- */
+int sfx_system_create(const sfx_lattice_desc* lat, const sfx_system_desc* sys,
+                      const sfx_options* opt, sfx_system** out);
+int sfx_system_destroy(sfx_system* s);
+/* Number of iteration variables (length of x and f) and of lattice sites M. */
+int sfx_system_sizes(const sfx_system* s, long* nvar, long* M);
+
+/* ---- hot path ---------------------------------------------------------- */
+/* u -> SWF -> propagators -> phi, ln G_N (UpdateSWF + MolQ->ComputePhi). */
+int sfx_compute_phi(sfx_system* s, const double* x, int where, int precision);
+/* One residual evaluation f(x) (SF_Solve::residuals / MPS ComputeG). */
+int sfx_residual(sfx_system* s, const double* x, double* f, int where, int precision);
+/* Jacobian-vector product J(x) v of the residual, by tangent propagators. */
+int sfx_jvp(sfx_system* s, const double* x, const double* v, double* jv, int where,
+            int precision);
+/* Transposed product J(x)^T w, by adjoint propagators. */
+int sfx_vjp(sfx_system* s, const double* x, const double* w, double* jtw, int where,
+            int precision);
+/* Results of the last sfx_compute_phi / sfx_residual. */
+int sfx_get_phi(sfx_system* s, int state, double* phi_out /* M */, int where);
+int sfx_get_lngn(sfx_system* s, int molecule, double* lngn);
+int sfx_get_phibulk(sfx_system* s, int molecule, double* phibulk);
+int sfx_free_energy(sfx_system* s, double* F);
+
+/* ---- solvers ----------------------------------------------------------- */
+enum { SFX_SOLVER_DIIS = 0, SFX_SOLVER_NK = 1 };
+
+typedef struct sfx_solver_opts {
+  int method;             /* SFX_SOLVER_DIIS (MPS.cpp:228-284) or SFX_SOLVER_NK */
+  double tolerance;       /* stop when ||f||_2 (FP64 residual) < tolerance */
+  int max_iter;           /* outer iterations */
+  int diis_m;             /* DIIS memory (MPS.cpp:306: 10) */
+  double diis_eta;        /* DIIS step (MPS.cpp:13: 0.1) */
+  int gmres_restart;      /* NK: GMRES restart length */
+  int gmres_max_iter;     /* NK: Krylov iterations per Newton step */
+  double gmres_rtol;      /* NK: relative Krylov tolerance (forcing term) */
+  int jvp_precision;      /* NK: precision of the tangent sweeps */
+  double fp64_switch;     /* NK: ||f|| below which residual sweeps run in FP64 */
+  int verbose;
+} sfx_solver_opts;
+
+typedef struct sfx_solve_stats {
+  int iterations;         /* outer iterations */
+  int residual_evals;
+  int residual_evals_fp32;
+  int jvp_evals;
+  double residual_norm;   /* final ||f||_2 of the FP64 residual */
+  double seconds;         /* wall time of the solve (HIP events) */
+  int converged;
+} sfx_solve_stats;
+
+int sfx_solver_defaults(sfx_solver_opts* o);
+/* Solves f(x)=0 from x (in/out, host or device per `where`). */
+int sfx_solve(sfx_system* s, const sfx_solver_opts* o, double* x, int where,
+              sfx_solve_stats* stats);
+
+/* ---- per-rank lattice primitives (Cuda_tools.h:3-26 / Lat1stO.h) -------- */
+/* All arrays are device pointers of M doubles; stream may be NULL. */
+typedef struct sfx_lattice sfx_lattice;
+int sfx_lattice_create(const sfx_lattice_desc* d, sfx_lattice** out);
+int sfx_lattice_destroy(sfx_lattice* l);
+int sfx_lattice_sites(const sfx_lattice* l, long* M);
+/* Gout = G * stencil(Gin), halos of Gout filled (Lat1stO::PropagateG(Gi,G,Gout)). */
+int sfx_lattice_propagate(const sfx_lattice* l, const double* Gin, const double* G,
+                          double* Gout, void* stream);
+/* out += A * B on every site (Lat1stO::ConnectG(A,B,out)). */
+int sfx_lattice_connect(const sfx_lattice* l, const double* A, const double* B,
+                        double* out, void* stream);
+/* in /= G where G > 0, else 0 (CorrectDoubleCountG). */
+int sfx_lattice_correct_double_count(const sfx_lattice* l, double* in, const double* G,
+                                     void* stream);
+/* ln of the site-weighted sum of G over the interior (ComputeLnGN). */
+int sfx_lattice_lngn(const sfx_lattice* l, const double* G, double* lngn, void* stream);
+/* Fills the halo layers of P (SetBoundaries). */
+int sfx_lattice_set_boundaries(const sfx_lattice* l, double* P, void* stream);
+/* Contact average <P> (SideFraction on every site / Lat3D::Sides). */
+int sfx_lattice_sides(const sfx_lattice* l, const double* P, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SFX_H */

@@ -1,0 +1,79 @@
+/*
+ * orc.h -- ORACLE (test infrastructure only). FP64 CPU restatement of the
+ * reference's Scheutjens-Fleer hot path. Loaded by tests/, smoke() (as the
+ * checker) and bench.py's cpu_baseline leg only; never by the product path.
+ */
+#ifndef ORC_H
+#define ORC_H
+
+#include "../include/sfx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_lat {
+  int grad, geom, type, order;
+  int n[3];
+  int MX, MY, MZ;
+  long M, jx, jy;
+  int code1[3], code2[3]; /* boundary choice per gradient (SFX_BOUND_*) */
+  int b1[3], b2[3];       /* halo source coordinate per gradient (0-based) */
+  double l1, l1chi, ladj, offset, dimensions;
+  int adjust_outer;
+  double l0_2d, l1x, l1y;
+  double *L, *lm, *lp, *lmc, *lpc; /* per-x-layer tables of curved lattices */
+} orc_lat;
+
+orc_lat* orc_lat_new(const sfx_lattice_desc* d);
+void orc_lat_free(orc_lat* L);
+long orc_lat_M(const orc_lat* L);
+int orc_lat_is_interior(const orc_lat* L, long i);
+void orc_setb(const orc_lat* L, double* P);
+void orc_zero_halos(const orc_lat* L, double* P);
+void orc_propagate(const orc_lat* L, double* Gin, const double* g, double* Gout);
+void orc_subtract_boundaries(const orc_lat* L, double* P);
+void orc_multiply_sites(const orc_lat* L, double* P);
+double orc_lngn(const orc_lat* L, const double* G);
+void orc_side(const orc_lat* L, double* P, double* out);
+
+/* ---- SF system ---- */
+typedef struct orc_sys orc_sys;
+orc_sys* orc_sys_new(const sfx_lattice_desc* ld, const sfx_system_desc* sd);
+void orc_sys_free(orc_sys* s);
+long orc_sys_nvar(const orc_sys* s);
+long orc_sys_M(const orc_sys* s);
+int orc_sys_n_itvar(const orc_sys* s);
+/* UpdateSWF + ComputePhi + GetResiduals; returns 0 or a negative code. */
+int orc_sys_residual(orc_sys* s, const double* x, double* f);
+int orc_sys_compute_phi(orc_sys* s, const double* x);
+void orc_sys_get_phi(const orc_sys* s, int state, double* out);
+void orc_sys_get_swf(const orc_sys* s, int state, double* out);
+double orc_sys_lngn(const orc_sys* s, int mol);
+double orc_sys_phibulk(const orc_sys* s, int mol);
+double orc_sys_free_energy(orc_sys* s);
+/* Homogeneous initial guess (x = 0 everywhere). */
+int orc_sys_solve_diis(orc_sys* s, double* x, double tol, int max_iter, int m, double eta,
+                       int* iterations, double* final_norm);
+
+/* ---- MPS.cpp literal restatement (3-D SC periodic AB melt) ---- */
+typedef struct orc_mps orc_mps;
+orc_mps* orc_mps_new(int mx, int my, int mz, int n1, int n2, double chi, double n_chains,
+                     double phib_a, double phib_b);
+void orc_mps_free(orc_mps* p);
+long orc_mps_M(const orc_mps* p);
+/* ComputeG at u = x (x: 2*M, halos zero); g out: 2*M. */
+void orc_mps_compute_g(orc_mps* p, const double* x, double* g);
+void orc_mps_get_phi(const orc_mps* p, double* phi /* 2*M */);
+double orc_mps_gn(const orc_mps* p);
+double orc_mps_helmholtz(orc_mps* p, const double* x);
+int orc_mps_scf(orc_mps* p, double* x, double tol, int max_iter, int m, double eta,
+                int* iterations, double* final_error);
+
+/* small dense symmetric solve used by DIIS: X = A^{-1} 1 (MPS.cpp:200-226) */
+void orc_diis_coeffs(const double* A, int n, double* X);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

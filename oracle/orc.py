@@ -1,0 +1,215 @@
+"""ORACLE binding (test infrastructure only).
+
+ctypes access to liborc.so, the FP64 CPU restatement of the reference's SCF
+hot path. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg import this module: as the checker, or as the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+D = C.c_double
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        L.orc_sys_new.restype = C.c_void_p
+        L.orc_sys_new.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_sys_free.argtypes = [C.c_void_p]
+        L.orc_sys_nvar.restype = C.c_long
+        L.orc_sys_nvar.argtypes = [C.c_void_p]
+        L.orc_sys_M.restype = C.c_long
+        L.orc_sys_M.argtypes = [C.c_void_p]
+        L.orc_sys_residual.argtypes = [C.c_void_p, PD, PD]
+        L.orc_sys_compute_phi.argtypes = [C.c_void_p, PD]
+        L.orc_sys_get_phi.argtypes = [C.c_void_p, C.c_int, PD]
+        L.orc_sys_get_swf.argtypes = [C.c_void_p, C.c_int, PD]
+        L.orc_sys_lngn.restype = D
+        L.orc_sys_lngn.argtypes = [C.c_void_p, C.c_int]
+        L.orc_sys_phibulk.restype = D
+        L.orc_sys_phibulk.argtypes = [C.c_void_p, C.c_int]
+        L.orc_sys_free_energy.restype = D
+        L.orc_sys_free_energy.argtypes = [C.c_void_p]
+        L.orc_sys_solve_diis.argtypes = [C.c_void_p, PD, D, C.c_int, C.c_int, D, PI, PD]
+        L.orc_mps_new.restype = C.c_void_p
+        L.orc_mps_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D, D, D, D]
+        L.orc_mps_free.argtypes = [C.c_void_p]
+        L.orc_mps_M.restype = C.c_long
+        L.orc_mps_M.argtypes = [C.c_void_p]
+        L.orc_mps_compute_g.argtypes = [C.c_void_p, PD, PD]
+        L.orc_mps_get_phi.argtypes = [C.c_void_p, PD]
+        L.orc_mps_gn.restype = D
+        L.orc_mps_gn.argtypes = [C.c_void_p]
+        L.orc_mps_helmholtz.restype = D
+        L.orc_mps_helmholtz.argtypes = [C.c_void_p, PD]
+        L.orc_mps_scf.argtypes = [C.c_void_p, PD, D, C.c_int, C.c_int, D, PI, PD]
+        L.orc_lat_new.restype = C.c_void_p
+        L.orc_lat_new.argtypes = [C.c_void_p]
+        L.orc_lat_free.argtypes = [C.c_void_p]
+        L.orc_lat_M.restype = C.c_long
+        L.orc_lat_M.argtypes = [C.c_void_p]
+        L.orc_setb.argtypes = [C.c_void_p, PD]
+        L.orc_propagate.argtypes = [C.c_void_p, PD, PD, PD]
+        L.orc_lngn.restype = D
+        L.orc_lngn.argtypes = [C.c_void_p, PD]
+        L.orc_side.argtypes = [C.c_void_p, PD, PD]
+        L.orc_subtract_boundaries.argtypes = [C.c_void_p, PD]
+        L.orc_diis_coeffs.argtypes = [PD, C.c_int, PD]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(PD)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Lattice:
+    def __init__(self, spec):
+        self.spec = spec
+        self._d = spec.desc()
+        self.h = lib().orc_lat_new(C.byref(self._d))
+        self.M = lib().orc_lat_M(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_lat_free(self.h)
+
+    def propagate(self, Gin, g):
+        Gin = _f64(Gin).copy()
+        out = np.zeros(self.M)
+        lib().orc_propagate(self.h, _p(Gin), _p(_f64(g)), _p(out))
+        return out
+
+    def set_boundaries(self, P):
+        P = _f64(P).copy()
+        lib().orc_setb(self.h, _p(P))
+        return P
+
+    def lngn(self, G):
+        return lib().orc_lngn(self.h, _p(_f64(G)))
+
+    def sides(self, P):
+        P = _f64(P).copy()
+        out = np.zeros(self.M)
+        lib().orc_side(self.h, _p(P), _p(out))
+        return out
+
+
+class System:
+    """Oracle SCF system built from field_amd.spec descriptions."""
+
+    def __init__(self, lat_spec, sys_spec):
+        self.lat, self.sys = lat_spec, sys_spec
+        self._ld = lat_spec.desc()
+        self._sd, self._keep = sys_spec.desc(lat_spec)
+        self.h = lib().orc_sys_new(C.byref(self._ld), C.byref(self._sd))
+        if not self.h:
+            raise ValueError("oracle rejected the system description")
+        self.nvar = lib().orc_sys_nvar(self.h)
+        self.M = lib().orc_sys_M(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_sys_free(self.h)
+
+    def residual(self, x):
+        x = _f64(x)
+        f = np.zeros(self.nvar)
+        rc = lib().orc_sys_residual(self.h, _p(x), _p(f))
+        if rc:
+            raise FloatingPointError(f"oracle residual returned {rc}")
+        return f
+
+    def compute_phi(self, x):
+        lib().orc_sys_compute_phi(self.h, _p(_f64(x)))
+
+    def phi(self, state):
+        out = np.zeros(self.M)
+        lib().orc_sys_get_phi(self.h, state, _p(out))
+        return out
+
+    def swf(self, state):
+        out = np.zeros(self.M)
+        lib().orc_sys_get_swf(self.h, state, _p(out))
+        return out
+
+    def lngn(self, mol):
+        return lib().orc_sys_lngn(self.h, mol)
+
+    def phibulk(self, mol):
+        return lib().orc_sys_phibulk(self.h, mol)
+
+    def free_energy(self):
+        return lib().orc_sys_free_energy(self.h)
+
+    def solve_diis(self, x, tol=1e-7, max_iter=1000, m=10, eta=0.1):
+        x = _f64(x).copy()
+        it = C.c_int(0)
+        err = C.c_double(0)
+        rc = lib().orc_sys_solve_diis(self.h, _p(x), tol, max_iter, m, eta, C.byref(it), C.byref(err))
+        return x, it.value, err.value, rc
+
+
+class MPS:
+    """Literal restatement of MPS.cpp (3-D periodic AB melt)."""
+
+    def __init__(self, n, n1, n2, chi, n_chains, phib_a=0.0, phib_b=0.0):
+        self.h = lib().orc_mps_new(n, n, n, n1, n2, chi, n_chains, phib_a, phib_b)
+        self.M = lib().orc_mps_M(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mps_free(self.h)
+
+    def residual(self, x):
+        g = np.zeros(2 * self.M)
+        lib().orc_mps_compute_g(self.h, _p(_f64(x)), _p(g))
+        return g
+
+    def phi(self):
+        out = np.zeros(2 * self.M)
+        lib().orc_mps_get_phi(self.h, _p(out))
+        return out
+
+    def gn(self):
+        return lib().orc_mps_gn(self.h)
+
+    def helmholtz(self, x):
+        return lib().orc_mps_helmholtz(self.h, _p(_f64(x)))
+
+    def scf(self, x, tol=1e-7, max_iter=1000, m=10, eta=0.1):
+        x = _f64(x).copy()
+        it = C.c_int(0)
+        err = C.c_double(0)
+        rc = lib().orc_mps_scf(self.h, _p(x), tol, max_iter, m, eta, C.byref(it), C.byref(err))
+        return x, it.value, err.value, rc
+
+
+def diis_coeffs(A):
+    A = _f64(A)
+    n = A.shape[0]
+    X = np.zeros(n)
+    lib().orc_diis_coeffs(_p(A), n, _p(X))
+    return X

@@ -1,0 +1,163 @@
+"""GPU (HIP, gfx950) against the CPU oracle on the same inputs, through the C ABI.
+
+Tolerances: FP64 path 1e-10 relative (absolute floor 1e-12 for entries near
+zero); FP32 propagators with per-rank log-scaling are reported against the
+FP64 oracle and bounded at 2e-4 absolute on residual entries."""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+ATOL = 1e-12
+
+
+def _close(a, b, rtol=RTOL, atol=ATOL):
+    np.testing.assert_allclose(a, b, rtol=rtol, atol=atol)
+
+
+@pytest.fixture(scope="module")
+def built(sfx, orc):
+    out = {}
+    for name in cases.NAMES:
+        lat, sysd = cases.case(name)
+        out[name] = (lat, sysd, sfx.System(lat, sysd), orc.System(lat, sysd))
+    return out
+
+
+@pytest.mark.parametrize("name", cases.NAMES)
+def test_residual_phi_lngn_fp64(built, name):
+    lat, sysd, G, O = built[name]
+    print(f"case {name} nvar={G.nvar}", flush=True)
+    x = cases.start(lat, sysd)
+    fg = G.residual(x, precision="fp64")
+    fo = O.residual(x)
+    _close(fg, fo)
+    for t in range(len(sysd.states)):
+        _close(G.phi(t), O.phi(t))
+    for m in range(len(sysd.molecules)):
+        assert abs(G.lngn(m) - O.lngn(m)) <= RTOL * max(1.0, abs(O.lngn(m)))
+        assert abs(G.phibulk(m) - O.phibulk(m)) <= RTOL * max(1e-3, abs(O.phibulk(m)))
+
+
+@pytest.mark.parametrize("name", cases.NAMES)
+def test_free_energy_fp64(built, name):
+    lat, sysd, G, O = built[name]
+    x = cases.start(lat, sysd, seed=12)
+    G.residual(x)
+    O.residual(x)
+    Fg, Fo = G.free_energy(), O.free_energy()
+    assert abs(Fg - Fo) <= 1e-10 * max(1.0, abs(Fo)), (Fg, Fo)
+
+
+@pytest.mark.parametrize("name", cases.NAMES)
+def test_fp32_logscaled_against_fp64_oracle(built, name):
+    lat, sysd, G, O = built[name]
+    x = cases.start(lat, sysd, seed=13)
+    f32 = G.residual(x, precision="fp32")
+    fo = O.residual(x)
+    err = np.abs(f32 - fo).max()
+    print(f"{name}: FP32 propagators, max |f32 - f64| = {err:.3e}")
+    assert err < 2e-4
+    lg = G.lngn(0)
+    assert abs(lg - O.lngn(0)) < 1e-4 * max(1, abs(O.lngn(0)))
+
+
+def test_fp32_scaling_survives_underflow(sfx, orc):
+    """A long chain on a strongly repulsive field: G_N ~ e^-N*u drops below the
+    FP32 range (1e-38) without the per-rank log-scale; with it, ln G_N and phi
+    stay accurate."""
+    from field_amd.spec import LatticeSpec, MoleculeSpec, SystemSpec
+    lat = LatticeSpec(1, (40,), lowerbound=("mirror1",), upperbound=("mirror1",), lambda_=1 / 6)
+    sysd = SystemSpec(("A", "S"), np.array([[0, 0.5], [0.5, 0]]),
+                      [MoleculeSpec("p", "(A)300", "restricted", theta=4.0), MoleculeSpec("s", "(S)1", "solvent")])
+    G, O = sfx.System(lat, sysd), orc.System(lat, sysd)
+    x = np.full(G.nvar, 0.6) + np.random.default_rng(3).normal(0, 0.1, G.nvar)
+    f32 = G.residual(x, precision="fp32")
+    fo = O.residual(x)
+    assert O.lngn(0) < math.log(1e-38)  # the unscaled FP32 propagator would underflow
+    assert abs(G.lngn(0) - O.lngn(0)) < 1e-4 * abs(O.lngn(0))
+    assert np.abs(f32 - fo).max() < 1e-3
+
+
+@pytest.mark.parametrize("name", cases.NAMES)
+def test_jvp_matches_central_differences(built, name):
+    """J v by tangent propagators against (f(x+hv) - f(x-hv)) / 2h, h = 1e-6, FP64."""
+    lat, sysd, G, O = built[name]
+    rng = np.random.default_rng(21)
+    x = cases.start(lat, sysd, seed=14)
+    for k in range(2):
+        v = rng.normal(0, 1, G.nvar)
+        jv = G.jvp(x, v, precision="fp64")
+        h = 1e-6
+        fd = (O.residual(x + h * v) - O.residual(x - h * v)) / (2 * h)
+        err = np.abs(jv - fd).max() / max(1e-12, np.abs(fd).max())
+        print(f"{name}: JVP vs central FD rel err {err:.2e}")
+        assert err < 1e-6
+        jv32 = G.jvp(x, v, precision="fp32")
+        err32 = np.abs(jv32 - jv).max() / np.abs(jv).max()
+        print(f"{name}: FP32 JVP vs FP64 JVP rel err {err32:.2e}")
+        assert err32 < 1e-3
+
+
+def test_diis_follows_the_oracle_trajectory(sfx, orc):
+    from field_amd import configs
+    lat, sysd, x = configs.melt_3d(n=6, n_a=3, n_b=5, chi=0.4, sigma=0.1, seed=7)
+    G, O = sfx.System(lat, sysd), orc.System(lat, sysd)
+    xg, st = G.solve(x, method="diis", tolerance=1e-8)
+    xo, it, err, rc = O.solve_diis(x, tol=1e-8)
+    assert rc == 0 and st["converged"]
+    assert abs(st["iterations"] - it) <= 2, (st["iterations"], it)
+    fo = O.residual(xg)
+    assert np.linalg.norm(fo) < 1.5e-8
+
+
+@pytest.mark.parametrize("name", ["1d_flat_adsorption", "2d_flat", "3d_sc_melt", "3d_hex_three_molecules"])
+def test_newton_krylov_converges(built, name, orc):
+    lat, sysd, G, O = built[name]
+    x = cases.start(lat, sysd, seed=15, sigma=0.05)
+    xs, st = G.solve(x, method="nk", tolerance=1e-9, max_iter=60, gmres_rtol=1e-4)
+    print(f"{name}: NK {st}")
+    assert st["converged"]
+    f = O.residual(xs)  # the oracle's FP64 residual at the GPU solution
+    assert np.linalg.norm(f) < 1e-8
+    G.residual(xs)
+    assert abs(G.free_energy() - O.free_energy()) <= 1e-10 * max(1, abs(O.free_energy()))
+
+
+def test_lattice_primitives_match_oracle(sfx, orc):
+    """Cuda_tools-style per-rank calls on device pointers."""
+    import ctypes as C
+    import torch
+    from field_amd.spec import LatticeSpec
+    lat = LatticeSpec(3, (5, 6, 7), lattice_type="fcc", lowerbound=("periodic", "mirror1", "mirror2"),
+                      upperbound=("periodic", "mirror2", "mirror1"))
+    h = C.c_void_p()
+    d = lat.desc()
+    sfx.check(sfx.lib().sfx_lattice_create(C.byref(d), C.byref(h)))
+    rng = np.random.default_rng(5)
+    G = rng.random(lat.M)
+    g = rng.random(lat.M)
+    OL = orc.Lattice(lat)
+    Gs = OL.set_boundaries(G)
+    ref = OL.propagate(Gs, OL.set_boundaries(g))
+    tG = torch.tensor(Gs, device="cuda")
+    tg = torch.tensor(OL.set_boundaries(g), device="cuda")
+    tout = torch.zeros(lat.M, dtype=torch.float64, device="cuda")
+    sfx.check(sfx.lib().sfx_lattice_propagate(h, C.c_void_p(tG.data_ptr()), C.c_void_p(tg.data_ptr()),
+                                              C.c_void_p(tout.data_ptr()), None))
+    torch.cuda.synchronize()
+    _close(tout.cpu().numpy(), ref)
+    ln = C.c_double()
+    sfx.check(sfx.lib().sfx_lattice_lngn(h, C.c_void_p(tout.data_ptr()), C.byref(ln), None))
+    assert abs(ln.value - OL.lngn(ref)) < 1e-12
+    side = torch.zeros_like(tout)
+    sfx.check(sfx.lib().sfx_lattice_sides(h, C.c_void_p(tG.data_ptr()), C.c_void_p(side.data_ptr()), None))
+    torch.cuda.synchronize()
+    m = lat.interior_mask()
+    _close(side.cpu().numpy()[m], OL.sides(Gs)[m])
+    sfx.lib().sfx_lattice_destroy(h)
