@@ -94,3 +94,17 @@ class SolveStats(C.Structure):
         ("seconds", C.c_double),
         ("converged", C.c_int),
     ]
+
+
+K_FWD, K_BWD, K_TFWD, K_TBWD = 0, 1, 2, 3
+KERNEL_NAMES = {K_FWD: "propagate", K_BWD: "propagate+compose", K_TFWD: "tangent propagate",
+                K_TBWD: "tangent propagate+compose"}
+
+
+class ProfileEntry(C.Structure):
+    _fields_ = [("kind", C.c_int), ("precision", C.c_int), ("launches", C.c_long), ("total_ms", C.c_double),
+                ("bytes_per_launch", C.c_double)]
+
+
+class ProfileReport(C.Structure):
+    _fields_ = [("count", C.c_int), ("entries", ProfileEntry * 8)]

@@ -16,6 +16,7 @@
 #include <cmath>
 
 #include "sfx_device.h"
+#include "sfx_order2.h"
 #include "sfx_solve.h"
 
 namespace sfx {
@@ -30,8 +31,9 @@ struct Tangent {
   void init(System& S) {
     M = S.lat.d.M;
     maxstore = S.maxstore;
-    dstore.alloc((size_t)maxstore * M);
-    dpp.alloc(2 * (size_t)M);
+    const size_t comps = S.lat.d.order == 2 ? 6 : 1;
+    dstore.alloc((size_t)maxstore * comps * M);
+    dpp.alloc(2 * comps * (size_t)M);
     dacc.alloc((size_t)S.nm * S.ns * M);
     dphi_m.alloc((size_t)S.nm * S.ns * M);
     dphi.alloc((size_t)S.ns * M);
@@ -68,7 +70,7 @@ __device__ __forceinline__ T vdir(const DevLattice& L, const T* v, int x, int y,
 template <typename T>
 __global__ void k_tinit(DevLattice L, const T* __restrict__ g, const T* __restrict__ v, T* __restrict__ G,
                         T* __restrict__ dG, RankScale sc) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   float m = 0.f;
   if (q < L.V) {
     int x, y, z;
@@ -100,7 +102,7 @@ __global__ void __launch_bounds__(kTB) k_tprop(DevLattice L, const T* __restrict
                                                T* __restrict__ Gout, T* __restrict__ dGout,
                                                const T* __restrict__ Gp, const T* __restrict__ dGp,
                                                T* __restrict__ dacc, RankScale sc) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   T scale = T(1), cfac = T(1);
   double lncum = 0;
   if constexpr (SCALED) {
@@ -142,7 +144,7 @@ __global__ void __launch_bounds__(kTB) k_tprop(DevLattice L, const T* __restrict
 template <typename T, bool SCALED>
 __global__ void k_tconnect(DevLattice L, const T* __restrict__ A, const T* __restrict__ dA, const T* __restrict__ B,
                            const T* __restrict__ dB, T* __restrict__ dacc, RankScale sc, double half) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   if (q >= L.V) return;
   T c = T(half);
   if constexpr (SCALED) c = (T)(half * exp(sc.lnv[sc.step] + sc.lnpair[sc.pair] - *sc.ref));
@@ -156,7 +158,7 @@ __global__ void k_tconnect(DevLattice L, const T* __restrict__ A, const T* __res
 template <typename T>
 __global__ void k_wsum2(DevLattice L, const T* __restrict__ G, const T* __restrict__ dG, double* __restrict__ p1,
                         double* __restrict__ p2) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   double a = 0, b = 0;
   if (q < L.V) {
     int x, y, z;
@@ -190,7 +192,7 @@ __global__ void k_wsum2(DevLattice L, const T* __restrict__ G, const T* __restri
 // monomer: sum w g and sum w (-v g)
 __global__ void k_wsum_mono(DevLattice L, const double* __restrict__ g, const double* __restrict__ v,
                             double* __restrict__ p1, double* __restrict__ p2) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   double a = 0, b = 0;
   if (q < L.V) {
     int x, y, z;
@@ -239,7 +241,7 @@ __global__ void k_tmol_finalize(MolParams mp, const double* p1, const double* p2
     lnpref = log(mp.theta) - ms[MS_LNGN] - (mp.kind == 0 ? 0.0 : log((double)mp.N));
   else
     lnpref = mp.kind == 0 ? log(phib) : log(phib / mp.N);
-  d[3] = (mp.kind == 2 ? 2.0 : 1.0) * exp(lnpref + ms[MS_REF]);
+  d[3] = (mp.kind == 2 ? 2.0 : mp.kind == 3 ? 1.0 / 6.0 : 1.0) * exp(lnpref + ms[MS_REF]);
   d[0] = dln;
   if (mp.freedom == SFX_RESTRICTED) {
     d[1] = -dln;
@@ -261,7 +263,7 @@ template <typename T>
 __global__ void k_tmol_phi(DevLattice L, const double* __restrict__ phi_m, const T* __restrict__ dacc,
                            const double* __restrict__ g, const double* __restrict__ v, const double* __restrict__ scal,
                            const double* __restrict__ dm, int monomer, double* __restrict__ dphi_m) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   if (q >= L.V) return;
   int x, y, z;
   interior_coords(L, q, x, y, z);
@@ -303,7 +305,7 @@ __global__ void k_tphi_total(DevLattice L, int nstates, int nmol, const double* 
 }
 
 __global__ void k_tsides(DevLattice L, int nstates, const double* __restrict__ dphi, double* __restrict__ drho) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   if (q >= L.V) return;
   int x, y, z;
   interior_coords(L, q, x, y, z);
@@ -417,6 +419,7 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
   auto rank = [&](int s) { return store + (size_t)(s - 1) * Mn; };
   auto drank = [&](int s) { return dstore + (size_t)(s - 1) * Mn; };
   const int N = mo.N;
+  const int P = sizeof(T) == 4 ? SFX_FP32 : SFX_FP64;
   for (int t = 0; t < S.ns; t++)
     if (mo.uses[t]) zero<T>(daccp(t), Mn, st);
   RankScale none{nullptr, nullptr, nullptr, nullptr, 0, 0};
@@ -429,7 +432,34 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
   const T* fin = nullptr;
   const T* dfin = nullptr;
   double* dm = W.dmol.p;
-  if (mo.kind == 0) {
+  if (mo.kind == 3) {
+    const long long M6 = 6 * Mn;
+    T* pp6[2] = {(T*)S.d_pp, (T*)S.d_pp + M6};
+    T* dpp6[2] = {(T*)W.dpp.p, (T*)W.dpp.p + M6};
+    auto rank6 = [&](int s) { return store + (size_t)(s - 1) * M6; };
+    auto drank6 = [&](int s) { return dstore + (size_t)(s - 1) * M6; };
+    const double Sst = S.stiffness;
+    launch_init2<T>(L, true, gp(mo.seq[0]), vp(mo.seq[0]), rank6(1), drank6(1), nullptr, nullptr, nullptr,
+                    fsc(1, 0), st);
+    for (int s = 2; s <= N; s++) {
+      S.prof.begin(2, P, st);
+      launch_prop2<T>(L, false, true, rank6(s - 1), drank6(s - 1), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), rank6(s),
+                      drank6(s), nullptr, nullptr, nullptr, Sst, fsc(s, 0), st);
+      S.prof.end(st);
+    }
+    k_set_ref2<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
+    int tN = mo.seq[N - 1];
+    launch_init2<T>(L, true, gp(tN), vp(tN), pp6[0], dpp6[0], rank6(N), drank6(N), daccp(tN), bsc(1, N), st);
+    for (int s = N - 1; s >= 1; s--) {
+      int k = N - s + 1, t = mo.seq[s - 1];
+      S.prof.begin(3, P, st);
+      launch_prop2<T>(L, true, true, pp6[(k - 2) & 1], dpp6[(k - 2) & 1], gp(t), vp(t), pp6[(k - 1) & 1],
+                      dpp6[(k - 1) & 1], rank6(s), drank6(s), daccp(t), Sst, bsc(k, s), st);
+      S.prof.end(st);
+    }
+    int nb = launch_wsum6<T>(L, pp6[(N - 1) & 1], dpp6[(N - 1) & 1], S.d_partial, W.partial2.p, st);
+    k_tmol_finalize<<<1, 1, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, nb, S.d_molscal, dm, S.nm);
+  } else if (mo.kind == 0) {
     int t = mo.seq[0];
     unsigned nb = grid1(L.V, kTB);
     k_wsum_mono<<<nb, kTB, 0, st>>>(L, S.d_swf + (size_t)t * Mn, v64 + (size_t)t * Mn, S.d_partial, W.partial2.p);
@@ -437,9 +467,12 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
   } else {
     k_tinit<T><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), rank(1), drank(1), fsc(1, 0));
     if (mo.kind == 1) {
-      for (int s = 2; s <= N; s++)
+      for (int s = 2; s <= N; s++) {
+        S.prof.begin(2, P, st);
         tprop<T>(L, rank(s - 1), drank(s - 1), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), rank(s), drank(s), nullptr,
                  nullptr, nullptr, fsc(s, 0), st);
+        S.prof.end(st);
+      }
       k_set_ref2<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
       int tN = mo.seq[N - 1];
       k_tinit<T><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(tN), vp(tN), pp[0], dpp[0], bsc(1, N));
@@ -450,16 +483,21 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
       for (int s = N - 1; s >= 1; s--) {
         int k = N - s + 1;
         int t = mo.seq[s - 1];
+        S.prof.begin(3, P, st);
         tprop<T>(L, pp[(k - 2) & 1], dpp[(k - 2) & 1], gp(t), vp(t), pp[(k - 1) & 1], dpp[(k - 1) & 1], rank(s),
                  drank(s), daccp(t), bsc(k, s), st);
+        S.prof.end(st);
       }
       fin = pp[(N - 1) & 1];
       dfin = dpp[(N - 1) & 1];
     } else {
       const int h = N / 2;
-      for (int s = 2; s <= h; s++)
+      for (int s = 2; s <= h; s++) {
+        S.prof.begin(2, P, st);
         tprop<T>(L, rank(s - 1), drank(s - 1), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), rank(s), drank(s), nullptr,
                  nullptr, nullptr, fsc(s, 0), st);
+        S.prof.end(st);
+      }
       k_set_ref2<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, h, 2.0);
       launch_copy_interior<T>(L, rank(h), pp[0], st);
       launch_copy_interior<T>(L, drank(h), dpp[0], st);
@@ -476,8 +514,10 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
       }
       for (int s = (N + 3) / 2; s <= N; s++) {
         int sinv = N - s + 1, t = mo.seq[s - 1];
+        S.prof.begin(3, P, st);
         tprop<T>(L, pp[cur], dpp[cur], gp(t), vp(t), pp[cur ^ 1], dpp[cur ^ 1], rank(sinv), drank(sinv), daccp(t),
                  fsc(s, sinv), st);
+        S.prof.end(st);
         cur ^= 1;
       }
       fin = pp[cur];

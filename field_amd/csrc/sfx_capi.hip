@@ -6,6 +6,7 @@
 #include <new>
 #include <string>
 
+#include "../../include/sfx_profile.h"
 #include "sfx_solve.h"
 
 namespace sfx {
@@ -198,6 +199,46 @@ int sfx_solve(sfx_system* s, const sfx_solver_opts* o, double* x, int where, sfx
     S.store_f(S.d_u, x, where);
     if (stats) *stats = st;
     if (!st.converged) throw Error{SFX_E_NOTCONVERGED, "solver reached its iteration limit"};
+  });
+}
+
+// ---- kernel timing (include/sfx_profile.h) ----------------------------------
+int sfx_profile_start(sfx_system* s) {
+  return guarded([&] {
+    SFX_REQUIRE(s, SFX_E_INVALID, "NULL system");
+    Profiler& p = s->s.prof;
+    p.marks.clear();
+    p.used = 0;
+    p.on = true;
+  });
+}
+
+int sfx_profile_stop(sfx_system* s, sfx_profile_report* rep) {
+  return guarded([&] {
+    SFX_REQUIRE(s && rep, SFX_E_INVALID, "NULL argument");
+    System& S = s->s;
+    Profiler& p = S.prof;
+    S.sync();
+    p.on = false;
+    std::memset(rep, 0, sizeof *rep);
+    const double words[SFX_K_COUNT] = {3, 6, 6, 10};
+    for (int k = 0; k < SFX_K_COUNT; k++)
+      for (int pr = 0; pr < 2; pr++) {
+        sfx_profile_entry& e = rep->entries[k * 2 + pr];
+        e.kind = k;
+        e.precision = pr;
+        e.bytes_per_launch = words[k] * (pr == SFX_FP32 ? 4.0 : 8.0) * (double)S.lat.d.V;
+      }
+    for (const Profiler::Mark& m : p.marks) {
+      float ms = 0;
+      SFX_HIP(hipEventElapsedTime(&ms, p.pool[m.ev], p.pool[m.ev + 1]));
+      sfx_profile_entry& e = rep->entries[m.kind * 2 + m.prec];
+      e.launches++;
+      e.total_ms += ms;
+    }
+    rep->count = 2 * SFX_K_COUNT;
+    p.marks.clear();
+    p.used = 0;
   });
 }
 

@@ -9,22 +9,41 @@ __device__ __forceinline__ long long sidx(const DevLattice& L, int x, int y, int
   return ((long long)x * L.MY + y) * L.MZ + z;
 }
 
-// interior ordinal q in [0, V) -> coordinates (last used gradient fastest)
-__device__ __forceinline__ void interior_coords(const DevLattice& L, long long q, int& x, int& y, int& z) {
+// interior ordinal q in [0, V) -> coordinates (last used gradient fastest);
+// 32-bit arithmetic (V < 2^31 is checked when the lattice is created)
+__device__ __forceinline__ void interior_coords(const DevLattice& L, long long q64, int& x, int& y, int& z) {
+  const unsigned q = (unsigned)q64;
   if (L.grad == 1) {
     x = (int)q + 1;
     y = 0;
     z = 0;
   } else if (L.grad == 2) {
-    y = (int)(q % L.ny) + 1;
-    x = (int)(q / L.ny) + 1;
+    unsigned r = q / (unsigned)L.ny;
+    y = (int)(q - r * (unsigned)L.ny) + 1;
+    x = (int)r + 1;
     z = 0;
   } else {
-    z = (int)(q % L.nz) + 1;
-    long long r = q / L.nz;
-    y = (int)(r % L.ny) + 1;
-    x = (int)(r / L.ny) + 1;
+    unsigned r = q / (unsigned)L.nz;
+    z = (int)(q - r * (unsigned)L.nz) + 1;
+    unsigned r2 = r / (unsigned)L.ny;
+    y = (int)(r - r2 * (unsigned)L.ny) + 1;
+    x = (int)r2 + 1;
   }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X. This
+// maps the dispatch index to a tile index so that each XCD walks one
+// contiguous range of the lattice (its own L2 then holds the planes the next
+// rank reads); a bijection on [0, nb).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb) {
+  const unsigned per = nb >> 3;
+  if (b >= (per << 3)) return b;
+  return (b & 7u) * per + (b >> 3);
+}
+
+// thread ordinal over the interior of the current launch (XCD-aware)
+__device__ __forceinline__ long long site_ordinal() {
+  return (long long)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
 }
 
 __device__ __forceinline__ void site_coords(const DevLattice& L, long long i, int& x, int& y, int& z) {

@@ -73,6 +73,8 @@ struct DevLattice {
   int b1[3], b2[3];   // halo source coordinate per gradient
   int surf1[3], surf2[3];
   int mirror2_1[3], mirror2_2[3];
+  int periodic[3];
+  int order;
   double l1, l1chi, ladj;
   double l0_2d, l1x, l1y;
   // per-x-layer tables (device), curved lattices only

@@ -25,7 +25,7 @@ struct RankScale {
 };
 
 struct MolParams {
-  int index, N, freedom, kind;  // kind: 0 monomer, 1 full sweep, 2 half-stored (palindrome)
+  int index, N, freedom, kind;  // kind: 0 monomer, 1 full sweep, 2 half-stored (palindrome), 3 second order
   double phib, theta;
 };
 

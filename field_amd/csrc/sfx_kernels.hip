@@ -73,7 +73,7 @@ __global__ void k_swf(DevLattice L, const double* __restrict__ u, const unsigned
 // rank 1: G(1) = g (interior), optional running max for the FP32 log-scale
 template <typename T>
 __global__ void k_init_rank(DevLattice L, const T* __restrict__ g, T* __restrict__ out, RankScale sc) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   float m = 0.f;
   if (q < L.V) {
     int x, y, z;
@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(kBlock) k_prop(DevLattice L, const T* __restri
                                                  const T* __restrict__ g, T* __restrict__ Gout,
                                                  const T* __restrict__ Gpair, A* __restrict__ acc,
                                                  RankScale sc) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   T scale = T(1);
   double lncum = 0;
   T cfac = T(1);
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(kBlock) k_prop(DevLattice L, const T* __restri
 template <typename T, typename A, bool SCALED>
 __global__ void k_connect(DevLattice L, const T* __restrict__ Ga, const T* __restrict__ Gb,
                           A* __restrict__ acc, RankScale sc, double half) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   if (q >= L.V) return;
   T cfac = T(half);
   if constexpr (SCALED) cfac = (T)(half * exp(sc.lnv[sc.step] + sc.lnpair[sc.pair] - *sc.ref));
@@ -150,7 +150,7 @@ __global__ void k_connect(DevLattice L, const T* __restrict__ Ga, const T* __res
 
 template <typename T>
 __global__ void k_copy_interior(DevLattice L, const T* __restrict__ a, T* __restrict__ b) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   if (q >= L.V) return;
   int x, y, z;
   interior_coords(L, q, x, y, z);
@@ -161,7 +161,7 @@ __global__ void k_copy_interior(DevLattice L, const T* __restrict__ a, T* __rest
 // weighted interior sum of G (ComputeLnGN before the log), per-block partials
 template <typename T>
 __global__ void k_wsum(DevLattice L, const T* __restrict__ G, double* __restrict__ partial) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   double v = 0;
   if (q < L.V) {
     int x, y, z;
@@ -181,7 +181,7 @@ __global__ void k_mol_finalize(MolParams mp, const double* __restrict__ partial,
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double s = 0;
   for (int i = 0; i < nparts; i++) s += partial[i];
-  double lnsum = log(s);
+  double lnsum = mp.kind == 3 ? log(s / 6.0) : log(s);  // second order: Lat3D2ndO.cpp:1024
   double lnscale = lnfN ? *lnfN : 0.0;  // FP32: log-scale of the final rank
   double lnGN = lnsum + lnscale;
   double ref = scal[MS_REF];
@@ -200,7 +200,8 @@ __global__ void k_mol_finalize(MolParams mp, const double* __restrict__ partial,
     lnC = mp.kind == 0 ? log(phib) : log(phib / N);
     lnpref = lnC;
   }
-  double f = mp.kind == 2 ? 2.0 : 1.0;  // half-stored chains (SymMatrix1 factor 2)
+  // half-stored chains (SymMatrix1 factor 2); second order divides by 6 (Lat3D2ndO.cpp:1037)
+  double f = mp.kind == 2 ? 2.0 : mp.kind == 3 ? 1.0 / 6.0 : 1.0;
   scal[MS_LNGN] = lnGN;
   scal[MS_LNC] = lnC;
   scal[MS_PHIB] = phib;
@@ -211,7 +212,7 @@ __global__ void k_mol_finalize(MolParams mp, const double* __restrict__ partial,
 template <typename A>
 __global__ void k_mol_phi(DevLattice L, const A* __restrict__ acc, const double* __restrict__ g,
                           const double* __restrict__ scal, int monomer, double* __restrict__ phi_m) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long q = site_ordinal();
   if (q >= L.V) return;
   int x, y, z;
   interior_coords(L, q, x, y, z);

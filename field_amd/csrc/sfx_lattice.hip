@@ -110,6 +110,7 @@ void lattice_init(Lattice& L, const sfx_lattice_desc& d) {
   D.jx = (long long)D.MY * D.MZ;
   D.jy = D.MZ;
   D.V = (long long)D.nx * D.ny * D.nz;
+  SFX_REQUIRE(D.M < (1LL << 31), SFX_E_UNSUPPORTED, "lattices are limited to 2^31 sites");
   for (int k = 0; k < 3; k++) {
     D.b1[k] = D.b2[k] = 0;
     if (k >= D.grad) continue;
@@ -125,7 +126,12 @@ void lattice_init(Lattice& L, const sfx_lattice_desc& d) {
     D.surf2[k] = up == SFX_BOUND_SURFACE;
     D.mirror2_1[k] = lo == SFX_BOUND_MIRROR2;
     D.mirror2_2[k] = up == SFX_BOUND_MIRROR2;
+    D.periodic[k] = lo == SFX_BOUND_PERIODIC;
   }
+  D.order = d.order == 2 ? 2 : 1;
+  if (D.order == 2)
+    SFX_REQUIRE(D.grad == 3 && d.lattice_type == SFX_LAT_STANDARD, SFX_E_UNSUPPORTED,
+                "second-order chains are built for the 3-D simple-cubic lattice");
   D.ladj = d.layer_adjustment > 0 ? d.layer_adjustment : 1.0;
   D.l1 = d.lambda;
   D.l1chi = d.lambda_chi > 0 ? d.lambda_chi : d.lambda;

@@ -8,7 +8,7 @@
 namespace sfx {
 
 struct Molecule {
-  int N = 1, freedom = SFX_FREE, kind = 1;  // kind: 0 monomer, 1 full sweep, 2 half-stored
+  int N = 1, freedom = SFX_FREE, kind = 1;  // kind: 0 monomer, 1 full sweep, 2 half-stored, 3 second order
   std::vector<int> seq;
   std::vector<int> uses;  // per state: 1 if the chain contains it
   double phib = 0, theta = 0, chem_int_ref = 0;
@@ -18,6 +18,21 @@ struct Molecule {
   }
 };
 
+// HIP-event timing of propagator launches (include/sfx_profile.h)
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  struct Mark {
+    int kind, prec;
+    size_t ev;
+  };
+  std::vector<Mark> marks;
+  void begin(int kind, int prec, hipStream_t st);
+  void end(hipStream_t st);
+  ~Profiler();
+};
+
 struct Tangent;  // JVP / VJP buffers (sfx_adjoint.hip)
 void tangent_free(Tangent* t);
 
@@ -25,6 +40,7 @@ struct System {
   Lattice lat;
   int ns = 0, nm = 0, form = SFX_FORM_SFBOX, n_itvar = 0, precision = SFX_FP64;
   size_t budget = 0;
+  double stiffness = 0;
   std::vector<double> chi, fprof, melt_phib;
   std::vector<int> frozen, st_itvar, itvar_rep;
   std::vector<Molecule> mols;
@@ -65,6 +81,7 @@ struct System {
   double* d_xabi = nullptr;     // nvar: ABI-layout staging
   int npart = 0;
   Tangent* tan = nullptr;
+  Profiler prof;
   bool phi_valid = false;
   int last_precision = SFX_FP64;
 

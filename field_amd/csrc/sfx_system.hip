@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "sfx_order2.h"
 #include "sfx_system.h"
 
 namespace sfx {
@@ -76,7 +77,8 @@ static int is_palindrome(const std::vector<int>& s) {
 
 void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const sfx_options* opt) {
   lattice_init(lat, ld);
-  SFX_REQUIRE(ld.order != 2, SFX_E_UNSUPPORTED, "second-order (semiflexible) systems use sfx_system2_create");
+  stiffness = sd.stiffness;
+  if (lat.d.order == 2) SFX_REQUIRE(stiffness >= 0 && stiffness <= 1, SFX_E_INVALID, "stiffness S must be in [0, 1]");
   const long long Mn = lat.d.M;
   ns = sd.n_states;
   nm = sd.n_molecules;
@@ -162,6 +164,7 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
     if (d.freedom == SFX_FREE) SFX_REQUIRE(d.phibulk > 0, SFX_E_INVALID, "free molecule needs phibulk > 0");
     bool homo = std::all_of(mo.seq.begin(), mo.seq.end(), [&](int t) { return t == mo.seq[0]; });
     mo.kind = mo.N == 1 ? 0 : ((homo || is_palindrome(mo.seq)) ? 2 : 1);
+    if (lat.d.order == 2 && mo.kind != 0) mo.kind = 3;  // full six-component sweep (SF_Homopol2ndO.cpp:152-207)
     mo.nstore = mo.kind == 0 ? 0 : mo.kind == 2 ? mo.N / 2 : mo.N;
     maxN = std::max(maxN, mo.N);
     maxstore = std::max(maxstore, mo.nstore);
@@ -223,9 +226,10 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
   SFX_REQUIRE(budget == 0 || (size_t)maxstore * M * 8 <= budget, SFX_E_UNSUPPORTED,
               "memory budget below the full propagator store: checkpointed sweeps are not built for this "
               "system type");
-  d_store = dalloc<double>((size_t)maxstore * M);  // sized for FP64 so both precisions fit
   (void)word;
-  d_pp = dalloc<double>(2 * M);
+  const size_t comps = lat.d.order == 2 ? 6 : 1;
+  d_store = dalloc<double>((size_t)maxstore * comps * M);  // sized for FP64 so both precisions fit
+  d_pp = dalloc<double>(2 * comps * M);
   d_molscal = dalloc<double>((size_t)nm * MS_COUNT);
   d_lnf = dalloc<double>((size_t)nm * (maxN + 2));
   d_lnb = dalloc<double>((size_t)nm * (maxN + 2));
@@ -248,8 +252,8 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
   if (!sites.empty())
     SFX_HIP(hipMemcpy(d_sites, sites.data(), sites.size() * sizeof(long long), hipMemcpyHostToDevice));
   // zero everything once: halo slots of propagator arrays are never written
-  zero_bytes(d_store, (size_t)maxstore * M * 8, stream);
-  zero_bytes(d_pp, 2 * M * 8, stream);
+  zero_bytes(d_store, (size_t)maxstore * comps * M * 8, stream);
+  zero_bytes(d_pp, 2 * comps * M * 8, stream);
   zero_bytes(d_u, n_itvar * M * 8, stream);
   zero_bytes(d_f, n_itvar * M * 8, stream);
   zero_bytes(d_acc, (size_t)nm * ns * M * 8, stream);
@@ -260,6 +264,25 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
 }
 
 void System::sync() { SFX_HIP(hipStreamSynchronize(stream)); }
+
+void Profiler::begin(int kind, int prec, hipStream_t st) {
+  if (!on) return;
+  while (pool.size() < used + 2) {
+    hipEvent_t e;
+    SFX_HIP(hipEventCreate(&e));
+    pool.push_back(e);
+  }
+  marks.push_back(Mark{kind, prec, used});
+  SFX_HIP(hipEventRecord(pool[used], st));
+  used += 2;
+}
+void Profiler::end(hipStream_t st) {
+  if (!on) return;
+  SFX_HIP(hipEventRecord(pool[marks.back().ev + 1], st));
+}
+Profiler::~Profiler() {
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
 
 void System::load_x(const double* x, int where) {
   const long long Mn = lat.d.M;
@@ -317,6 +340,7 @@ void System::sweep(int m, bool scaled) {
   };
   auto rank = [&](int s) { return store + (size_t)(s - 1) * Mn; };
   const int N = mo.N;
+  const int P = sizeof(T) == 4 ? SFX_FP32 : SFX_FP64;
   for (int t = 0; t < ns; t++)
     if (mo.uses[t]) zero_bytes(accp(t), (size_t)Mn * sizeof(A), st);
   if (scaled) {
@@ -332,6 +356,35 @@ void System::sweep(int m, bool scaled) {
   };
   const T* final_G = nullptr;
   const double* lnfinal = nullptr;
+  if (mo.kind == 3) {
+    // second order: six components per rank, full store
+    const long long M6 = 6 * Mn;
+    T* pp6[2] = {(T*)d_pp, (T*)d_pp + M6};
+    auto rank6 = [&](int s) { return store + (size_t)(s - 1) * M6; };
+    launch_init2<T>(L, false, gp(mo.seq[0]), nullptr, rank6(1), nullptr, nullptr, nullptr, nullptr, fsc(1, 0), st);
+    for (int s = 2; s <= N; s++) {
+      prof.begin(0, P, st);
+      launch_prop2<T>(L, false, false, rank6(s - 1), nullptr, gp(mo.seq[s - 1]), nullptr, rank6(s), nullptr,
+                      nullptr, nullptr, nullptr, stiffness, fsc(s, 0), st);
+      prof.end(st);
+    }
+    k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
+    launch_init2<T>(L, false, gp(mo.seq[N - 1]), nullptr, pp6[0], nullptr, rank6(N), nullptr, accp(mo.seq[N - 1]),
+                    bsc(1, N), st);
+    for (int s = N - 1; s >= 1; s--) {
+      int k = N - s + 1;
+      prof.begin(1, P, st);
+      launch_prop2<T>(L, true, false, pp6[(k - 2) & 1], nullptr, gp(mo.seq[s - 1]), nullptr, pp6[(k - 1) & 1],
+                      nullptr, rank6(s), nullptr, accp(mo.seq[s - 1]), stiffness, bsc(k, s), st);
+      prof.end(st);
+    }
+    int nb = launch_wsum6<T>(L, pp6[(N - 1) & 1], nullptr, d_partial, nullptr, st);
+    launch_mol_finalize(mo.params(m), d_partial, nb, scal, scaled ? lnb + N : nullptr, d_molscal, nm, st);
+    for (int t = 0; t < ns; t++)
+      if (mo.uses[t])
+        launch_mol_phi<A>(L, accp(t), d_swf + (size_t)t * Mn, scal, 0, d_phi_m + ((size_t)m * ns + t) * Mn, st);
+    return;
+  }
   if (mo.kind == 0) {
     int t = mo.seq[0];
     int nb = launch_wsum<double>(L, d_swf + (size_t)t * Mn, d_partial, st);
@@ -342,23 +395,31 @@ void System::sweep(int m, bool scaled) {
   }
   launch_init_rank<T>(L, gp(mo.seq[0]), rank(1), fsc(1, 0), st);
   if (mo.kind == 1) {
-    for (int s = 2; s <= N; s++)
+    for (int s = 2; s <= N; s++) {
+      prof.begin(0, P, st);
       launch_prop<T, A>(L, rank(s - 1), gp(mo.seq[s - 1]), rank(s), nullptr, nullptr, fsc(s, 0), st);
+      prof.end(st);
+    }
     k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
     // backward: step k = N-s+1 holds G_inv(s)
     launch_init_rank<T>(L, gp(mo.seq[N - 1]), pp[0], bsc(1, N), st);
     launch_connect<T, A>(L, pp[0], rank(N), accp(mo.seq[N - 1]), bsc(1, N), 1.0, st);
     for (int s = N - 1; s >= 1; s--) {
       int k = N - s + 1;
+      prof.begin(1, P, st);
       launch_prop<T, A>(L, pp[(k - 2) & 1], gp(mo.seq[s - 1]), pp[(k - 1) & 1], rank(s), accp(mo.seq[s - 1]),
                         bsc(k, s), st);
+      prof.end(st);
     }
     final_G = pp[(N - 1) & 1];
     lnfinal = scaled ? lnb + N : nullptr;
   } else {
     const int h = N / 2;
-    for (int s = 2; s <= h; s++)
+    for (int s = 2; s <= h; s++) {
+      prof.begin(0, P, st);
       launch_prop<T, A>(L, rank(s - 1), gp(mo.seq[s - 1]), rank(s), nullptr, nullptr, fsc(s, 0), st);
+      prof.end(st);
+    }
     k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, h, 2.0);
     // the running sweep continues the forward chain (ranks h+1..N)
     launch_copy_interior<T>(L, rank(h), pp[0], st);
@@ -372,8 +433,10 @@ void System::sweep(int m, bool scaled) {
     }
     for (int s = (N + 3) / 2; s <= N; s++) {
       int sinv = N - s + 1;
+      prof.begin(1, P, st);
       launch_prop<T, A>(L, pp[cur], gp(mo.seq[s - 1]), pp[cur ^ 1], rank(sinv), accp(mo.seq[s - 1]), fsc(s, sinv),
                         st);
+      prof.end(st);
       cur ^= 1;
     }
     final_G = pp[cur];

@@ -39,6 +39,8 @@ _SIGS = {
     "sfx_free_energy": (C.c_int, [C.c_void_p, PD]),
     "sfx_solver_defaults": (C.c_int, [C.c_void_p]),
     "sfx_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "sfx_profile_start": (C.c_int, [C.c_void_p]),
+    "sfx_profile_stop": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sfx_lattice_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "sfx_lattice_destroy": (C.c_int, [C.c_void_p]),
     "sfx_lattice_sites": (C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
@@ -184,6 +186,22 @@ class System:
         v = C.c_double()
         check(lib().sfx_free_energy(self.h, C.byref(v)))
         return v.value
+
+    def profile_start(self):
+        check(lib().sfx_profile_start(self.h))
+
+    def profile_stop(self):
+        """Per kernel kind and precision: launches, summed HIP-event ms, bytes per launch."""
+        rep = _abi.ProfileReport()
+        check(lib().sfx_profile_stop(self.h, C.byref(rep)))
+        out = []
+        for e in rep.entries[: rep.count]:
+            if e.launches:
+                out.append({"kernel": _abi.KERNEL_NAMES[e.kind], "kind": e.kind,
+                            "precision": "fp32" if e.precision == _abi.FP32 else "fp64",
+                            "launches": e.launches, "total_ms": e.total_ms,
+                            "avg_us": 1e3 * e.total_ms / e.launches, "bytes_per_launch": e.bytes_per_launch})
+        return out
 
     def solve(self, x0, method="diis", check_converged=True, **kw):
         o = _abi.SolverOpts()

@@ -1,0 +1,47 @@
+/*
+ * sfx_profile.h -- kernel timing of the propagator sweeps (diagnostic
+ * interface). It has no counterpart in the reference, which times nothing on
+ * the GPU (only clock() ticks in sf_vectoriterations.cpp:7-9): it is for
+ * callers that report achieved bandwidth, such as benchmark drivers.
+ *
+ * Between sfx_profile_start and sfx_profile_stop every propagator kernel of
+ * the system is bracketed by HIP events on the stream it runs on. The report
+ * sums the event durations per kernel kind and precision and gives the
+ * algorithmic bytes one launch moves (interior sites only):
+ *   SFX_K_FWD   G_s = g A[G_{s-1}]                      3 words / site
+ *   SFX_K_BWD   G_inv,s = g A[G_inv,s+1], phi += G G_inv 6 words / site
+ *   SFX_K_TFWD  base + tangent forward step              6 words / site
+ *   SFX_K_TBWD  base + tangent backward step + compose   10 words / site
+ * (a word is 8 bytes for FP64 and 4 bytes for FP32).
+ */
+#ifndef SFX_PROFILE_H
+#define SFX_PROFILE_H
+
+#include "sfx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { SFX_K_FWD = 0, SFX_K_BWD = 1, SFX_K_TFWD = 2, SFX_K_TBWD = 3, SFX_K_COUNT = 4 };
+
+typedef struct sfx_profile_entry {
+  int kind;                 /* SFX_K_* */
+  int precision;            /* SFX_FP64 / SFX_FP32 */
+  long launches;
+  double total_ms;          /* summed event durations */
+  double bytes_per_launch;  /* algorithmic bytes of one launch */
+} sfx_profile_entry;
+
+typedef struct sfx_profile_report {
+  int count;
+  sfx_profile_entry entries[2 * SFX_K_COUNT];
+} sfx_profile_report;
+
+int sfx_profile_start(sfx_system* s);
+int sfx_profile_stop(sfx_system* s, sfx_profile_report* report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SFX_PROFILE_H */

@@ -36,6 +36,10 @@ void orc_subtract_boundaries(const orc_lat* L, double* P);
 void orc_multiply_sites(const orc_lat* L, double* P);
 double orc_lngn(const orc_lat* L, const double* G);
 void orc_side(const orc_lat* L, double* P, double* out);
+/* second order (3-D simple cubic), six direction components per site */
+void orc_setb6(const orc_lat* L, double* P);
+void orc_propagate2(const orc_lat* L, double* Gin, const double* g, double* Gout, double S, int backward);
+double orc_lngn2(const orc_lat* L, const double* G);
 
 /* ---- SF system ---- */
 typedef struct orc_sys orc_sys;

@@ -47,6 +47,8 @@ struct orc_sys {
   double *swf, *phi, *rho, *phitot, *phib_state, *u;
   double *Gi, *Ginv, *Gtmp;
   int maxN;
+  int order;
+  double stiffness;
 };
 
 static int is_palindrome(const int* s, int N) {
@@ -143,9 +145,12 @@ orc_sys* orc_sys_new(const sfx_lattice_desc* ld, const sfx_system_desc* sd) {
   s->phitot = calloc(M, sizeof(double));
   s->phib_state = calloc(ns, sizeof(double));
   s->u = calloc(s->n_itvar * M, sizeof(double));
-  s->Gi = calloc((size_t)M * s->maxN, sizeof(double));
-  s->Ginv = calloc(M, sizeof(double));
-  s->Gtmp = calloc(M, sizeof(double));
+  s->order = ld->order == 2 ? 2 : 1;
+  s->stiffness = sd->stiffness;
+  int D = s->order == 2 ? 6 : 1;
+  s->Gi = calloc((size_t)M * D * s->maxN, sizeof(double));
+  s->Ginv = calloc(M * D, sizeof(double));
+  s->Gtmp = calloc(M * D, sizeof(double));
   return s;
 }
 
@@ -228,10 +233,52 @@ static void scale(orc_sys* s, double* phi, double c) {
   for (long z = 0; z < s->M; z++) phi[z] *= c;
 }
 
+/* SF_Homopol2ndO.cpp:152-207 (full sweep, six direction components) */
+static void mol_compute_phi2(orc_sys* s, omol* o) {
+  long M = s->M, M6 = 6 * s->M;
+  int N = o->N;
+  double S = s->stiffness;
+  double* Gi = s->Gi;
+  double* Ginv = s->Ginv;
+  for (int k = 0; k < 6; k++) memcpy(Gi + k * M, s->swf + o->seq[0] * M, sizeof(double) * M);
+  for (int r = 2; r <= N; r++)
+    orc_propagate2(s->L, Gi + (long)(r - 2) * M6, s->swf + o->seq[r - 1] * M, Gi + (long)(r - 1) * M6, S, 0);
+  for (int k = 0; k < 6; k++) memcpy(Ginv + k * M, s->swf + o->seq[N - 1] * M, sizeof(double) * M);
+  for (int q = N; q >= 1; q--) {
+    int t = o->seq[q - 1];
+    if (q < N) {
+      memcpy(s->Gtmp, Ginv, sizeof(double) * M6);
+      orc_propagate2(s->L, s->Gtmp, s->swf + t * M, Ginv, S, 1);
+    }
+    for (int k = 0; k < 6; k++) connect(s, o->phi + t * M, Ginv + k * M, Gi + (long)(q - 1) * M6 + k * M);
+  }
+  o->lnGN = orc_lngn2(s->L, Ginv);
+  for (int t = 0; t < s->ns; t++) {
+    int used = 0;
+    for (int k = 0; k < N; k++) used |= o->seq[k] == t;
+    if (used) correct_dc(s, o->phi + t * M, s->swf + t * M);
+  }
+  double c;
+  if (o->freedom == SFX_RESTRICTED) {
+    o->lnC = log(o->theta) - o->lnGN - log(1.0 * N);
+    o->phib = o->theta / exp(o->lnGN);
+    c = o->theta / N / (6.0 * exp(o->lnGN));
+  } else {
+    o->lnC = log(o->phib / N);
+    c = o->phib / (6.0 * N);
+  }
+  for (int t = 0; t < s->ns; t++) scale(s, o->phi + t * M, c);
+}
+
 static void mol_compute_phi(orc_sys* s, omol* o) {
   long M = s->M;
   int N = o->N;
   double* G;
+  if (s->order == 2 && o->kind != 0) {
+    memset(o->phi, 0, sizeof(double) * s->ns * M);
+    mol_compute_phi2(s, o);
+    return;
+  }
   memset(o->phi, 0, sizeof(double) * s->ns * M);
   if (o->kind == 0) { /* SF_Monomer1stO::Matrix1 */
     int t = o->seq[0];

@@ -1,5 +1,7 @@
 """Small systems covering every lattice, molecule architecture and freedom of
 the hot path; shared by the GPU parity tests and smoke()."""
+import math
+
 import numpy as np
 
 from field_amd import configs
@@ -85,11 +87,28 @@ def case(name):
                            MoleculeSpec("c", "(C)1", "restricted", theta=10.0),
                            MoleculeSpec("s", "(S)1", "solvent")])
         return lat, sysd
+    if name == "3d_sc_order2_homopolymer":
+        lat = LatticeSpec(3, (6, 5, 7), order=2, lowerbound=("periodic", "mirror1", "mirror2"),
+                          upperbound=("periodic", "mirror2", "mirror1"))
+        st = ("A", "S")
+        sysd = SystemSpec(st, _chi(st, [("A", "S", 0.6)]),
+                          [MoleculeSpec("p", "(A)12", "restricted", theta=60.0), MoleculeSpec("s", "(S)1", "solvent")],
+                          stiffness=1.0 / (1.0 + 4.0 * math.exp(-1.0)))
+        return lat, sysd
+    if name == "3d_sc_order2_copolymer":
+        lat = LatticeSpec(3, (5, 6, 4), order=2, lowerbound=("mirror1", "periodic", "mirror1"),
+                          upperbound=("mirror2", "periodic", "mirror1"))
+        st = ("A", "B", "S")
+        sysd = SystemSpec(st, _chi(st, [("A", "B", 0.9), ("A", "S", 0.3), ("B", "S", 0.5)]),
+                          [MoleculeSpec("p", "(A)3(B)5", "free", phibulk=0.1), MoleculeSpec("s", "(S)1", "solvent")],
+                          stiffness=0.3)
+        return lat, sysd
     raise KeyError(name)
 
 
 NAMES = ["1d_flat_adsorption", "1d_flat_odd_mirror2", "1d_cylinder", "1d_sphere", "2d_flat", "2d_cylinder",
-         "3d_sc_melt", "3d_fcc_restricted", "3d_hex_three_molecules"]
+         "3d_sc_melt", "3d_fcc_restricted", "3d_hex_three_molecules", "3d_sc_order2_homopolymer",
+         "3d_sc_order2_copolymer"]
 
 
 def start(lat, sysd, seed=11, sigma=0.2):

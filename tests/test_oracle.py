@@ -338,3 +338,44 @@ def test_contact_average_3d_sc_is_neighbour_mean(orc):
     Pp = _padded(P.reshape(4, 5, 3), lat)
     ref = sum(_shift(Pp, d) for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))) / 6
     np.testing.assert_allclose(out, ref.reshape(-1), rtol=1e-14)
+
+
+@pytest.mark.parametrize("bounds", [("periodic",) * 3, ("mirror1", "mirror2", "periodic")])
+def test_order2_restricted_mass_balance(orc, bounds):
+    """Second order: sum_r phi(r) = theta needs the backward sweep
+    (PropagateB) to be the adjoint of the forward one (PropagateF) including
+    the mirror component swaps (Lat3D.cpp:1276-1290)."""
+    lat = LatticeSpec(3, (5, 4, 6), order=2, lowerbound=bounds, upperbound=bounds[::-1])
+    sysd = SystemSpec(("A", "B", "S"), np.array([[0, 0.8, 0.3], [0.8, 0, 0.5], [0.3, 0.5, 0]]),
+                      [MoleculeSpec("p", "(A)4(B)7", "restricted", theta=30.0),
+                       MoleculeSpec("s", "(S)1", "solvent")], stiffness=0.45)
+    S = orc.System(lat, sysd)
+    x = np.random.default_rng(17).normal(0, 0.3, S.nvar)
+    S.compute_phi(x)
+    m = lat.interior_mask()
+    w = np.ones(lat.M)
+    MX, MY, MZ = lat.dims
+    w3 = w.reshape(MX, MY, MZ)
+    n = lat.n_layers
+    for ax in range(3):
+        lo, up = lat.lowerbound[ax], lat.upperbound[ax]
+        if "periodic" in (lo, up):
+            continue
+        for bound, layer in ((lo, 1), (up, n[ax])):
+            if bound == "mirror2":  # the mirror plane runs through this layer: half of it is inside
+                sl = [slice(None)] * 3
+                sl[ax] = layer
+                w3[tuple(sl)] *= 0.5
+    tot = np.sum((w * (S.phi(0) + S.phi(1)))[m])
+    assert abs(tot - 30.0) < 1e-11 * 30
+
+
+def test_order2_uniform_field_closed_form(orc):
+    lat = LatticeSpec(3, (4, 5, 3), order=2, lowerbound=("periodic",) * 3, upperbound=("periodic",) * 3)
+    sysd = SystemSpec(("A", "S"), np.array([[0, 0.6], [0.6, 0]]),
+                      [MoleculeSpec("p", "(A)9", "restricted", theta=12.0), MoleculeSpec("s", "(S)1", "solvent")],
+                      stiffness=0.404611)
+    S = orc.System(lat, sysd)
+    S.compute_phi(np.full(S.nvar, 0.25))
+    assert abs(S.lngn(0) - (math.log(60) - 9 * 0.25)) < 1e-12
+    np.testing.assert_allclose(S.phi(0)[lat.interior_mask()], 12.0 / 60, rtol=1e-12)
