@@ -84,6 +84,17 @@ class SolverOpts(C.Structure):
     ]
 
 
+class AdsorptionDesc(C.Structure):
+    _fields_ = [
+        ("n_layers", C.c_int),
+        ("N", C.c_int),
+        ("lambda_", C.c_double),
+        ("chi_as", C.c_double),
+        ("chi_s", C.c_double),
+        ("phibulk", C.c_double),
+    ]
+
+
 class SolveStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int),

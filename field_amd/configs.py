@@ -108,13 +108,20 @@ def cylinder_2d(nr=64, nz=128, sigma=0.05, seed=2):
 
 
 def batch_1d(count=512, seed0=4):
-    """Uneven 1-D adsorption problems: layers U{50..400}, N U{20..1000},
-    chi_AS U[0,1], chi_s U[-2,0], phi_b 10^U[-4,-1]; one seed per problem."""
+    """Parameters of uneven 1-D adsorption problems: layers U{50..400}, N
+    U{20..1000}, chi_AS U[0,1], chi_s U[-2,0], phi_b 10^U[-4,-1]; problem i
+    draws from seed seed0 + i. Each dict feeds sfx.Batch or, through
+    adsorption_problem, a single-system description of the same problem."""
     out = []
     for i in range(count):
         u = uniform(seed0 + i, 5)
-        layers = 50 + int(u[0] * 351)
-        N = 20 + int(u[1] * 981)
-        out.append(adsorption_1d(layers, N, phibulk=10 ** (-4 + 3 * u[4]),
-                                 chi_as=u[2], chi_s=-2 * u[3]))
+        out.append({"n_layers": 50 + int(u[0] * 351), "N": 20 + int(u[1] * 981),
+                    "chi_as": float(u[2]), "chi_s": float(-2 * u[3]),
+                    "phibulk": float(10 ** (-4 + 3 * u[4])), "lambda_": 1.0 / 6.0})
     return out
+
+
+def adsorption_problem(p):
+    """(lattice, system, zero start) of one batch_1d-style problem dict."""
+    return adsorption_1d(p["n_layers"], p["N"], phibulk=p["phibulk"], chi_as=p["chi_as"],
+                         chi_s=p["chi_s"], lam=p.get("lambda_", 1.0 / 6.0))

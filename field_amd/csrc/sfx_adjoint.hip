@@ -30,9 +30,9 @@ struct Tangent {
   DevBuf<double> adj_a, adj_b;  // VJP work arrays
   void init(System& S) {
     M = S.lat.d.M;
-    maxstore = S.maxstore;
+    maxstore = S.store_slots;
     const size_t comps = S.lat.d.order == 2 ? 6 : 1;
-    dstore.alloc((size_t)maxstore * comps * M);
+    dstore.alloc((size_t)maxstore * comps * M);  // the same slots as the propagator store
     dpp.alloc(2 * comps * (size_t)M);
     dacc.alloc((size_t)S.nm * S.ns * M);
     dphi_m.alloc((size_t)S.nm * S.ns * M);
@@ -236,12 +236,12 @@ __global__ void k_tmol_finalize(MolParams mp, const double* p1, const double* p2
   double dln = (p1 && n > 0) ? s2 / s1 : 0.0;
   const double* ms = molscal + mp.index * MS_COUNT;
   double phib = ms[MS_PHIB];
-  double lnpref;
+  // the prefactor of the tangent sweep's scaling (its own reference log-scale)
+  const double f = mp.kind == 2 ? 2.0 : mp.kind == 3 ? 1.0 / 6.0 : 1.0;
   if (mp.freedom == SFX_RESTRICTED)
-    lnpref = log(mp.theta) - ms[MS_LNGN] - (mp.kind == 0 ? 0.0 : log((double)mp.N));
+    d[3] = f * exp(log(mp.theta) - ms[MS_LNGN] - (mp.kind == 0 ? 0.0 : log((double)mp.N)) + ms[MS_REF]);
   else
-    lnpref = mp.kind == 0 ? log(phib) : log(phib / mp.N);
-  d[3] = (mp.kind == 2 ? 2.0 : mp.kind == 3 ? 1.0 / 6.0 : 1.0) * exp(lnpref + ms[MS_REF]);
+    d[3] = f * (mp.kind == 0 ? phib : phib / mp.N) * exp(ms[MS_REF]);
   d[0] = dln;
   if (mp.freedom == SFX_RESTRICTED) {
     d[1] = -dln;
@@ -416,8 +416,6 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
   auto daccp = [&](int t) { return dacc + ((size_t)m * S.ns + t) * Mn; };
   auto gp = [&](int t) { return g32or64 + (size_t)t * Mn; };
   auto vp = [&](int t) { return vt + (size_t)t * Mn; };
-  auto rank = [&](int s) { return store + (size_t)(s - 1) * Mn; };
-  auto drank = [&](int s) { return dstore + (size_t)(s - 1) * Mn; };
   const int N = mo.N;
   const int P = sizeof(T) == 4 ? SFX_FP32 : SFX_FP64;
   for (int t = 0; t < S.ns; t++)
@@ -432,29 +430,34 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
   const T* fin = nullptr;
   const T* dfin = nullptr;
   double* dm = W.dmol.p;
+  const ChainPlan& plan = mo.plan;
   if (mo.kind == 3) {
     const long long M6 = 6 * Mn;
     T* pp6[2] = {(T*)S.d_pp, (T*)S.d_pp + M6};
     T* dpp6[2] = {(T*)W.dpp.p, (T*)W.dpp.p + M6};
-    auto rank6 = [&](int s) { return store + (size_t)(s - 1) * M6; };
-    auto drank6 = [&](int s) { return dstore + (size_t)(s - 1) * M6; };
+    auto slot6 = [&](int i) { return store + (size_t)i * M6; };
+    auto dslot6 = [&](int i) { return dstore + (size_t)i * M6; };
     const double Sst = S.stiffness;
-    launch_init2<T>(L, true, gp(mo.seq[0]), vp(mo.seq[0]), rank6(1), drank6(1), nullptr, nullptr, nullptr,
+    launch_init2<T>(L, true, gp(mo.seq[0]), vp(mo.seq[0]), slot6(0), dslot6(0), nullptr, nullptr, nullptr,
                     fsc(1, 0), st);
-    for (int s = 2; s <= N; s++) {
+    auto step = [&](int s, int src, int dst) {
       S.prof.begin(2, P, st);
-      launch_prop2<T>(L, false, true, rank6(s - 1), drank6(s - 1), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), rank6(s),
-                      drank6(s), nullptr, nullptr, nullptr, Sst, fsc(s, 0), st);
+      launch_prop2<T>(L, false, true, slot6(src), dslot6(src), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), slot6(dst),
+                      dslot6(dst), nullptr, nullptr, nullptr, Sst, fsc(s, 0), st);
       S.prof.end(st);
-    }
+    };
+    chain_forward(plan, step);
+    ChainReader<decltype(step)> rd{plan, step};
     k_set_ref2<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
     int tN = mo.seq[N - 1];
-    launch_init2<T>(L, true, gp(tN), vp(tN), pp6[0], dpp6[0], rank6(N), drank6(N), daccp(tN), bsc(1, N), st);
+    const int iN = rd.get(N);
+    launch_init2<T>(L, true, gp(tN), vp(tN), pp6[0], dpp6[0], slot6(iN), dslot6(iN), daccp(tN), bsc(1, N), st);
     for (int s = N - 1; s >= 1; s--) {
       int k = N - s + 1, t = mo.seq[s - 1];
+      const int is = rd.get(s);
       S.prof.begin(3, P, st);
       launch_prop2<T>(L, true, true, pp6[(k - 2) & 1], dpp6[(k - 2) & 1], gp(t), vp(t), pp6[(k - 1) & 1],
-                      dpp6[(k - 1) & 1], rank6(s), drank6(s), daccp(t), Sst, bsc(k, s), st);
+                      dpp6[(k - 1) & 1], slot6(is), dslot6(is), daccp(t), Sst, bsc(k, s), st);
       S.prof.end(st);
     }
     int nb = launch_wsum6<T>(L, pp6[(N - 1) & 1], dpp6[(N - 1) & 1], S.d_partial, W.partial2.p, st);
@@ -465,42 +468,45 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
     k_wsum_mono<<<nb, kTB, 0, st>>>(L, S.d_swf + (size_t)t * Mn, v64 + (size_t)t * Mn, S.d_partial, W.partial2.p);
     k_tmol_finalize<<<1, 1, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, (int)nb, S.d_molscal, dm, S.nm);
   } else {
-    k_tinit<T><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), rank(1), drank(1), fsc(1, 0));
+    auto slot = [&](int i) { return store + (size_t)i * Mn; };
+    auto dslot = [&](int i) { return dstore + (size_t)i * Mn; };
+    k_tinit<T><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0));
+    auto step = [&](int s, int src, int dst) {
+      S.prof.begin(2, P, st);
+      tprop<T>(L, slot(src), dslot(src), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), slot(dst), dslot(dst), nullptr,
+               nullptr, nullptr, fsc(s, 0), st);
+      S.prof.end(st);
+    };
+    chain_forward(plan, step);  // ranks 2..N (full sweep) or 2..N/2 (half store)
+    ChainReader<decltype(step)> rd{plan, step};
     if (mo.kind == 1) {
-      for (int s = 2; s <= N; s++) {
-        S.prof.begin(2, P, st);
-        tprop<T>(L, rank(s - 1), drank(s - 1), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), rank(s), drank(s), nullptr,
-                 nullptr, nullptr, fsc(s, 0), st);
-        S.prof.end(st);
-      }
       k_set_ref2<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
       int tN = mo.seq[N - 1];
       k_tinit<T><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(tN), vp(tN), pp[0], dpp[0], bsc(1, N));
+      const int iN = rd.get(N);
       if (scaled)
-        k_tconnect<T, true><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[0], dpp[0], rank(N), drank(N), daccp(tN), bsc(1, N), 1.0);
+        k_tconnect<T, true><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[0], dpp[0], slot(iN), dslot(iN), daccp(tN),
+                                                             bsc(1, N), 1.0);
       else
-        k_tconnect<T, false><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[0], dpp[0], rank(N), drank(N), daccp(tN), bsc(1, N), 1.0);
+        k_tconnect<T, false><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[0], dpp[0], slot(iN), dslot(iN), daccp(tN),
+                                                              bsc(1, N), 1.0);
       for (int s = N - 1; s >= 1; s--) {
         int k = N - s + 1;
         int t = mo.seq[s - 1];
+        const int is = rd.get(s);
         S.prof.begin(3, P, st);
-        tprop<T>(L, pp[(k - 2) & 1], dpp[(k - 2) & 1], gp(t), vp(t), pp[(k - 1) & 1], dpp[(k - 1) & 1], rank(s),
-                 drank(s), daccp(t), bsc(k, s), st);
+        tprop<T>(L, pp[(k - 2) & 1], dpp[(k - 2) & 1], gp(t), vp(t), pp[(k - 1) & 1], dpp[(k - 1) & 1], slot(is),
+                 dslot(is), daccp(t), bsc(k, s), st);
         S.prof.end(st);
       }
       fin = pp[(N - 1) & 1];
       dfin = dpp[(N - 1) & 1];
     } else {
       const int h = N / 2;
-      for (int s = 2; s <= h; s++) {
-        S.prof.begin(2, P, st);
-        tprop<T>(L, rank(s - 1), drank(s - 1), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), rank(s), drank(s), nullptr,
-                 nullptr, nullptr, fsc(s, 0), st);
-        S.prof.end(st);
-      }
       k_set_ref2<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, h, 2.0);
-      launch_copy_interior<T>(L, rank(h), pp[0], st);
-      launch_copy_interior<T>(L, drank(h), dpp[0], st);
+      const int ih = rd.get(h);
+      launch_copy_interior<T>(L, slot(ih), pp[0], st);
+      launch_copy_interior<T>(L, dslot(ih), dpp[0], st);
       int cur = 0;
       if (N % 2 == 1) {
         int s = h + 1, t = mo.seq[s - 1];
@@ -514,8 +520,9 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
       }
       for (int s = (N + 3) / 2; s <= N; s++) {
         int sinv = N - s + 1, t = mo.seq[s - 1];
+        const int is = rd.get(sinv);
         S.prof.begin(3, P, st);
-        tprop<T>(L, pp[cur], dpp[cur], gp(t), vp(t), pp[cur ^ 1], dpp[cur ^ 1], rank(sinv), drank(sinv), daccp(t),
+        tprop<T>(L, pp[cur], dpp[cur], gp(t), vp(t), pp[cur ^ 1], dpp[cur ^ 1], slot(is), dslot(is), daccp(t),
                  fsc(s, sinv), st);
         S.prof.end(st);
         cur ^= 1;

@@ -35,6 +35,7 @@ struct Error {
 // runs f, turning exceptions into a status code and the thread's error message
 template <typename F>
 int guarded(F&& f) {
+  (void)hipGetLastError();  // an error an earlier call already reported must not resurface here
   try {
     f();
     return SFX_OK;

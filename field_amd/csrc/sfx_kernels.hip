@@ -192,20 +192,25 @@ __global__ void k_mol_finalize(MolParams mp, const double* __restrict__ partial,
     for (int m = 0; m < nmol; m++)
       if (m != mp.index) phib -= allscal[m * MS_COUNT + MS_PHIB];
   }
+  // half-stored chains (SymMatrix1 factor 2); second order divides by 6 (Lat3D2ndO.cpp:1037)
+  double f = mp.kind == 2 ? 2.0 : mp.kind == 3 ? 1.0 / 6.0 : 1.0;
+  double pref;
   if (mp.freedom == SFX_RESTRICTED) {
     lnC = log(mp.theta) - lnGN - (mp.kind == 0 ? 0.0 : log(N));
     phib = mp.theta / exp(lnGN);
-    lnpref = log(mp.theta) - lnGN - (mp.kind == 0 ? 0.0 : log(N));
-  } else {
-    lnC = mp.kind == 0 ? log(phib) : log(phib / N);
     lnpref = lnC;
+    pref = f * exp(lnpref + ref);
+  } else {
+    // phi = (phi_b / N) G G_inv / g: phi_b multiplies directly, since the
+    // solvent closure can make it negative (NormPhiFree, Lat1DFlat1stO.cpp:317-331)
+    const double C = mp.kind == 0 ? phib : phib / N;
+    lnC = log(C);  // NaN for phi_b <= 0, as ln of the reference's lnCb; used by the free energy only
+    pref = f * C * exp(ref);
   }
-  // half-stored chains (SymMatrix1 factor 2); second order divides by 6 (Lat3D2ndO.cpp:1037)
-  double f = mp.kind == 2 ? 2.0 : mp.kind == 3 ? 1.0 / 6.0 : 1.0;
   scal[MS_LNGN] = lnGN;
   scal[MS_LNC] = lnC;
   scal[MS_PHIB] = phib;
-  scal[MS_PREF] = f * exp(lnpref + ref);
+  scal[MS_PREF] = pref;
 }
 
 // phi_{m,t} = pref_m * acc_{m,t} / g_t (g>0, else 0); monomers phi = pref * g.

@@ -16,6 +16,7 @@ namespace sfx {
 
 constexpr int kDotBlock = 256;
 constexpr int kMaxDots = 16;
+constexpr int kMaxVecs = 32;  // VecList capacity
 
 // partial[d][block] = sum_i a_d[i] * b_d[i]
 static __global__ void __launch_bounds__(kDotBlock) k_multidot(DotList dl, long long n, double* partial) {
@@ -76,7 +77,7 @@ void VecOps::init(long long n_, hipStream_t st) {
   nblk = (int)std::min<long long>(grid1(n, kDotBlock), 1024);
   SFX_HIP(hipMalloc(&partial, sizeof(double) * kMaxDots * nblk));
   SFX_HIP(hipMalloc(&dres, sizeof(double) * kMaxDots));
-  SFX_HIP(hipMalloc(&dcoef, sizeof(double) * kMaxDots));
+  SFX_HIP(hipMalloc(&dcoef, sizeof(double) * kMaxVecs));
 }
 VecOps::~VecOps() {
   if (partial) (void)hipFree(partial);
@@ -100,6 +101,7 @@ double VecOps::dot(const double* a, const double* b) {
   return r;
 }
 void VecOps::combine(const VecList& vl, const double* host_c, double* y) {
+  SFX_REQUIRE(vl.count >= 1 && vl.count <= kMaxVecs, SFX_E_INVALID, "combination of 1..32 vectors");
   SFX_HIP(hipMemcpyAsync(dcoef, host_c, sizeof(double) * vl.count, hipMemcpyHostToDevice, stream));
   k_combine<<<grid1(n, 256), 256, 0, stream>>>(vl, dcoef, n, y);
 }

@@ -1,18 +1,82 @@
 // sfx_system.h -- one SCF system on one GPU (host side).
 #pragma once
 
+#include <algorithm>
 #include <memory>
 
 #include "sfx_kernels.h"
 
 namespace sfx {
 
+// Where the forward ranks 1..n of a chain live while the backward pass needs
+// them: every rank (c == 0), or every c-th rank (c ~ sqrt(n)) plus one segment
+// of up to c-1 ranks recomputed from its checkpoint when the backward pass
+// enters it. sqrt(n) checkpointing replaces Matrix1Long's cube-root scheme
+// (SF_Homopol1stO.cpp:49-51, 731-843): 2 sqrt(n) slots for one extra forward
+// pass.
+struct ChainPlan {
+  int n = 0, c = 0, nck = 0;
+  void init(int n_, bool checkpoint) {
+    n = n_;
+    c = 0;
+    nck = 0;
+    if (!checkpoint || n < 4) return;
+    c = 1;
+    while (c * c < n) c++;
+    nck = (n + c - 1) / c;
+  }
+  int slots() const { return c ? nck + std::max(2, c - 1) : n; }
+  int seg(int i) const { return nck + i; }
+};
+
+// Forward pass over ranks 2..n; rank 1 is already in slot 0.
+// step(s, src_slot, dst_slot) computes rank s from rank s-1.
+template <class Step>
+void chain_forward(const ChainPlan& P, Step&& step) {
+  if (!P.c) {
+    for (int s = 2; s <= P.n; s++) step(s, s - 2, s - 1);
+    return;
+  }
+  int prev = 0;
+  for (int s = 2; s <= P.n; s++) {
+    const int dst = (s - 1) % P.c == 0 ? (s - 1) / P.c : P.seg(s & 1);
+    step(s, prev, dst);
+    prev = dst;
+  }
+}
+
+// Slot of rank s for a backward pass that asks for ranks in non-increasing
+// order; recomputes the segment that holds s when it is not resident.
+template <class Step>
+struct ChainReader {
+  const ChainPlan& P;
+  Step& step;
+  int resident = -1;
+  int get(int s) {
+    if (!P.c) return s - 1;
+    const int j = (s - 1) / P.c, start = 1 + j * P.c;
+    if (s == start) return j;
+    if (resident != j) {
+      const int end = std::min(start + P.c - 1, P.n);
+      int prev = j;
+      for (int r = start + 1; r <= end; r++) {
+        const int dst = P.seg(r - start - 1);
+        step(r, prev, dst);
+        prev = dst;
+      }
+      resident = j;
+    }
+    return P.seg(s - start - 1);
+  }
+};
+
 struct Molecule {
   int N = 1, freedom = SFX_FREE, kind = 1;  // kind: 0 monomer, 1 full sweep, 2 half-stored, 3 second order
   std::vector<int> seq;
   std::vector<int> uses;  // per state: 1 if the chain contains it
   double phib = 0, theta = 0, chem_int_ref = 0;
-  int nstore = 0;         // ranks kept in the store
+  int nstore = 0;         // forward ranks the backward pass needs
+  ChainPlan plan;         // how they are kept
   MolParams params(int index) const {
     return MolParams{index, N, freedom, kind, phib, theta};
   }
@@ -48,6 +112,8 @@ struct System {
   std::vector<unsigned char> itmask;
   long long nvar = 0;
   int maxN = 1, maxstore = 1;
+  int store_slots = 1;     // rank slots of the propagator store (all molecules share it)
+  bool checkpointed = false;
   SysParams sp{};
   hipStream_t stream = nullptr;
 

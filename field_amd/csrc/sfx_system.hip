@@ -222,13 +222,20 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
   d_mol_uses = dalloc<int>((size_t)nm * ns);
   d_itmask = dalloc<unsigned char>(M);
   d_sites = dalloc<long long>(sites.size());
-  size_t word = precision == SFX_FP32 ? 4 : 8;
-  SFX_REQUIRE(budget == 0 || (size_t)maxstore * M * 8 <= budget, SFX_E_UNSUPPORTED,
-              "memory budget below the full propagator store: checkpointed sweeps are not built for this "
-              "system type");
-  (void)word;
+  // propagator store: every forward rank, or sqrt(N) checkpoints when the
+  // memory budget (bytes of one FP64 store) cannot hold them all
   const size_t comps = lat.d.order == 2 ? 6 : 1;
-  d_store = dalloc<double>((size_t)maxstore * comps * M);  // sized for FP64 so both precisions fit
+  const size_t slot_bytes = comps * M * sizeof(double);
+  checkpointed = budget != 0 && (size_t)maxstore * slot_bytes > budget;
+  store_slots = 1;
+  for (Molecule& mo : mols) {
+    mo.plan.init(mo.nstore, checkpointed);
+    store_slots = std::max(store_slots, mo.plan.slots());
+  }
+  if (checkpointed)
+    SFX_REQUIRE((size_t)store_slots * slot_bytes <= budget, SFX_E_OOM,
+                "memory budget below the checkpointed propagator store (2 sqrt(N) ranks)");
+  d_store = dalloc<double>((size_t)store_slots * comps * M);  // sized for FP64 so both precisions fit
   d_pp = dalloc<double>(2 * comps * M);
   d_molscal = dalloc<double>((size_t)nm * MS_COUNT);
   d_lnf = dalloc<double>((size_t)nm * (maxN + 2));
@@ -252,7 +259,7 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
   if (!sites.empty())
     SFX_HIP(hipMemcpy(d_sites, sites.data(), sites.size() * sizeof(long long), hipMemcpyHostToDevice));
   // zero everything once: halo slots of propagator arrays are never written
-  zero_bytes(d_store, (size_t)maxstore * comps * M * 8, stream);
+  zero_bytes(d_store, (size_t)store_slots * comps * M * 8, stream);
   zero_bytes(d_pp, 2 * comps * M * 8, stream);
   zero_bytes(d_u, n_itvar * M * 8, stream);
   zero_bytes(d_f, n_itvar * M * 8, stream);
@@ -338,7 +345,6 @@ void System::sweep(int m, bool scaled) {
     if constexpr (sizeof(T) == 4) return (const T*)(d_swf32 + (size_t)t * Mn);
     else return (const T*)(d_swf + (size_t)t * Mn);
   };
-  auto rank = [&](int s) { return store + (size_t)(s - 1) * Mn; };
   const int N = mo.N;
   const int P = sizeof(T) == 4 ? SFX_FP32 : SFX_FP64;
   for (int t = 0; t < ns; t++)
@@ -356,26 +362,30 @@ void System::sweep(int m, bool scaled) {
   };
   const T* final_G = nullptr;
   const double* lnfinal = nullptr;
+  const ChainPlan& plan = mo.plan;
   if (mo.kind == 3) {
-    // second order: six components per rank, full store
+    // second order: six components per rank
     const long long M6 = 6 * Mn;
     T* pp6[2] = {(T*)d_pp, (T*)d_pp + M6};
-    auto rank6 = [&](int s) { return store + (size_t)(s - 1) * M6; };
-    launch_init2<T>(L, false, gp(mo.seq[0]), nullptr, rank6(1), nullptr, nullptr, nullptr, nullptr, fsc(1, 0), st);
-    for (int s = 2; s <= N; s++) {
+    auto slot6 = [&](int i) { return store + (size_t)i * M6; };
+    launch_init2<T>(L, false, gp(mo.seq[0]), nullptr, slot6(0), nullptr, nullptr, nullptr, nullptr, fsc(1, 0), st);
+    auto step = [&](int s, int src, int dst) {
       prof.begin(0, P, st);
-      launch_prop2<T>(L, false, false, rank6(s - 1), nullptr, gp(mo.seq[s - 1]), nullptr, rank6(s), nullptr,
+      launch_prop2<T>(L, false, false, slot6(src), nullptr, gp(mo.seq[s - 1]), nullptr, slot6(dst), nullptr,
                       nullptr, nullptr, nullptr, stiffness, fsc(s, 0), st);
       prof.end(st);
-    }
+    };
+    chain_forward(plan, step);
+    ChainReader<decltype(step)> rd{plan, step};
     k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
-    launch_init2<T>(L, false, gp(mo.seq[N - 1]), nullptr, pp6[0], nullptr, rank6(N), nullptr, accp(mo.seq[N - 1]),
-                    bsc(1, N), st);
+    launch_init2<T>(L, false, gp(mo.seq[N - 1]), nullptr, pp6[0], nullptr, slot6(rd.get(N)), nullptr,
+                    accp(mo.seq[N - 1]), bsc(1, N), st);
     for (int s = N - 1; s >= 1; s--) {
       int k = N - s + 1;
+      const T* pair = slot6(rd.get(s));
       prof.begin(1, P, st);
       launch_prop2<T>(L, true, false, pp6[(k - 2) & 1], nullptr, gp(mo.seq[s - 1]), nullptr, pp6[(k - 1) & 1],
-                      nullptr, rank6(s), nullptr, accp(mo.seq[s - 1]), stiffness, bsc(k, s), st);
+                      nullptr, pair, nullptr, accp(mo.seq[s - 1]), stiffness, bsc(k, s), st);
       prof.end(st);
     }
     int nb = launch_wsum6<T>(L, pp6[(N - 1) & 1], nullptr, d_partial, nullptr, st);
@@ -393,21 +403,25 @@ void System::sweep(int m, bool scaled) {
     launch_mol_phi<double>(L, nullptr, d_swf + (size_t)t * Mn, scal, 1, d_phi_m + ((size_t)m * ns + t) * Mn, st);
     return;
   }
-  launch_init_rank<T>(L, gp(mo.seq[0]), rank(1), fsc(1, 0), st);
+  auto slot = [&](int i) { return store + (size_t)i * Mn; };
+  launch_init_rank<T>(L, gp(mo.seq[0]), slot(0), fsc(1, 0), st);
+  auto step = [&](int s, int src, int dst) {
+    prof.begin(0, P, st);
+    launch_prop<T, A>(L, slot(src), gp(mo.seq[s - 1]), slot(dst), nullptr, nullptr, fsc(s, 0), st);
+    prof.end(st);
+  };
+  chain_forward(plan, step);  // ranks 2..N (full sweep) or 2..N/2 (half store)
+  ChainReader<decltype(step)> rd{plan, step};
   if (mo.kind == 1) {
-    for (int s = 2; s <= N; s++) {
-      prof.begin(0, P, st);
-      launch_prop<T, A>(L, rank(s - 1), gp(mo.seq[s - 1]), rank(s), nullptr, nullptr, fsc(s, 0), st);
-      prof.end(st);
-    }
     k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
     // backward: step k = N-s+1 holds G_inv(s)
     launch_init_rank<T>(L, gp(mo.seq[N - 1]), pp[0], bsc(1, N), st);
-    launch_connect<T, A>(L, pp[0], rank(N), accp(mo.seq[N - 1]), bsc(1, N), 1.0, st);
+    launch_connect<T, A>(L, pp[0], slot(rd.get(N)), accp(mo.seq[N - 1]), bsc(1, N), 1.0, st);
     for (int s = N - 1; s >= 1; s--) {
       int k = N - s + 1;
+      const T* pair = slot(rd.get(s));
       prof.begin(1, P, st);
-      launch_prop<T, A>(L, pp[(k - 2) & 1], gp(mo.seq[s - 1]), pp[(k - 1) & 1], rank(s), accp(mo.seq[s - 1]),
+      launch_prop<T, A>(L, pp[(k - 2) & 1], gp(mo.seq[s - 1]), pp[(k - 1) & 1], pair, accp(mo.seq[s - 1]),
                         bsc(k, s), st);
       prof.end(st);
     }
@@ -415,14 +429,9 @@ void System::sweep(int m, bool scaled) {
     lnfinal = scaled ? lnb + N : nullptr;
   } else {
     const int h = N / 2;
-    for (int s = 2; s <= h; s++) {
-      prof.begin(0, P, st);
-      launch_prop<T, A>(L, rank(s - 1), gp(mo.seq[s - 1]), rank(s), nullptr, nullptr, fsc(s, 0), st);
-      prof.end(st);
-    }
     k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, h, 2.0);
     // the running sweep continues the forward chain (ranks h+1..N)
-    launch_copy_interior<T>(L, rank(h), pp[0], st);
+    launch_copy_interior<T>(L, slot(rd.get(h)), pp[0], st);
     int cur = 0;
     if (N % 2 == 1) {
       int s = h + 1;
@@ -433,9 +442,9 @@ void System::sweep(int m, bool scaled) {
     }
     for (int s = (N + 3) / 2; s <= N; s++) {
       int sinv = N - s + 1;
+      const T* pair = slot(rd.get(sinv));
       prof.begin(1, P, st);
-      launch_prop<T, A>(L, pp[cur], gp(mo.seq[s - 1]), pp[cur ^ 1], rank(sinv), accp(mo.seq[s - 1]), fsc(s, sinv),
-                        st);
+      launch_prop<T, A>(L, pp[cur], gp(mo.seq[s - 1]), pp[cur ^ 1], pair, accp(mo.seq[s - 1]), fsc(s, sinv), st);
       prof.end(st);
       cur ^= 1;
     }

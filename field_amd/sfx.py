@@ -41,6 +41,15 @@ _SIGS = {
     "sfx_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "sfx_profile_start": (C.c_int, [C.c_void_p]),
     "sfx_profile_stop": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sfx_batch_create": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "sfx_batch_destroy": (C.c_int, [C.c_void_p]),
+    "sfx_batch_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_void_p]),
+    "sfx_batch_tuning": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_void_p]),
+    "sfx_batch_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "sfx_batch_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    "sfx_batch_get_lngn": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sfx_batch_get_phi": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "sfx_lattice_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "sfx_lattice_destroy": (C.c_int, [C.c_void_p]),
     "sfx_lattice_sites": (C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
@@ -65,6 +74,10 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"{LIB_PATH} is missing: build the HIP library first (python -m field_amd.build)")
+        # torch ships its own HIP runtime; loading it first makes the library
+        # bind to that same runtime, so device pointers and streams are shared
+        import torch  # noqa: F401
+
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)
@@ -96,8 +109,13 @@ def _ptr(a):
     if isinstance(a, np.ndarray):
         assert a.dtype == np.float64 and a.flags.c_contiguous
         return C.c_void_p(a.ctypes.data), _abi.HOST
-    # torch tensor on the GPU
+    # torch tensor: the library runs on its own stream, so work torch queued
+    # on the tensor's stream must be finished before the library reads it
     assert a.dtype.__str__() == "torch.float64" and a.is_contiguous()
+    if a.is_cuda:
+        import torch
+
+        torch.cuda.current_stream(a.device).synchronize()
     return C.c_void_p(a.data_ptr()), (_abi.DEVICE if a.is_cuda else _abi.HOST)
 
 
@@ -220,3 +238,86 @@ class System:
         if rc != 0 and not (rc == _abi.SFX_E_NOTCONVERGED and not check_converged):
             raise SFError(rc, last_error())
         return x, stats
+
+
+class Batch:
+    """Many 1-D adsorption problems solved at once (sfx_batch_create).
+
+    `problems` is a sequence of dicts with keys n_layers, N, chi_as, chi_s,
+    phibulk and optionally lambda_ (default 1/6)."""
+
+    def __init__(self, problems):
+        self.problems = [dict(p) for p in problems]
+        arr = (_abi.AdsorptionDesc * len(self.problems))()
+        for a, p in zip(arr, self.problems):
+            a.n_layers, a.N = int(p["n_layers"]), int(p["N"])
+            a.lambda_ = float(p.get("lambda_", 1.0 / 6.0))
+            a.chi_as, a.chi_s, a.phibulk = float(p["chi_as"]), float(p["chi_s"]), float(p["phibulk"])
+        self._arr = arr
+        opt = _abi.Options()
+        opt.precision = _abi.FP64
+        h = C.c_void_p()
+        check(lib().sfx_batch_create(arr, len(self.problems), C.byref(opt), C.byref(h)))
+        self.h = h
+        nv = C.c_longlong()
+        offs = np.zeros(len(self.problems) + 1, dtype=np.int64)
+        check(lib().sfx_batch_sizes(self.h, C.byref(nv), C.c_void_p(offs.ctypes.data)))
+        self.nvar, self.offsets = nv.value, offs
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sfx_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def tuning(self):
+        th = C.c_int()
+        ms = (C.c_float * 3)()
+        check(lib().sfx_batch_tuning(self.h, C.byref(th), ms))
+        return th.value, list(ms)
+
+    def split(self, v):
+        return [v[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.problems))]
+
+    def residual(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64) if isinstance(x, np.ndarray) else x
+        f = np.empty(self.nvar) if isinstance(x, np.ndarray) else x.new_empty(self.nvar)
+        px, where = _ptr(x)
+        pf, _ = _ptr(f)
+        check(lib().sfx_batch_residual(self.h, px, pf, where))
+        return f
+
+    def solve(self, x0=None, check_converged=False, **kw):
+        """DIIS on every problem; returns (x, iterations, norms, stats)."""
+        o = _abi.SolverOpts()
+        check(lib().sfx_solver_defaults(C.byref(o)))
+        o.method = _abi.SOLVER_DIIS
+        for k, v in kw.items():
+            setattr(o, k, v)
+        if x0 is None:
+            x = np.zeros(self.nvar)
+        elif isinstance(x0, np.ndarray):
+            x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        else:
+            x = x0.clone()
+        its = np.zeros(len(self.problems), dtype=np.int32)
+        norms = np.zeros(len(self.problems))
+        st = _abi.SolveStats()
+        px, where = _ptr(x)
+        rc = lib().sfx_batch_solve(self.h, C.byref(o), px, where, C.c_void_p(its.ctypes.data),
+                                   C.c_void_p(norms.ctypes.data), C.byref(st))
+        stats = {k: getattr(st, k) for k, _ in _abi.SolveStats._fields_}
+        if rc != 0 and not (rc == _abi.SFX_E_NOTCONVERGED and not check_converged):
+            raise SFError(rc, last_error())
+        return x, its, norms, stats
+
+    def lngn(self):
+        out = np.zeros(len(self.problems))
+        check(lib().sfx_batch_get_lngn(self.h, C.c_void_p(out.ctypes.data)))
+        return out
+
+    def phi(self, problem, state):
+        out = np.zeros(self.problems[problem]["n_layers"] + 2)
+        check(lib().sfx_batch_get_phi(self.h, problem, state, C.c_void_p(out.ctypes.data)))
+        return out

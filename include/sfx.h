@@ -188,6 +188,52 @@ int sfx_solver_defaults(sfx_solver_opts* o);
 int sfx_solve(sfx_system* s, const sfx_solver_opts* o, double* x, int where,
               sfx_solve_stats* stats);
 
+/* ---- batched 1-D adsorption solves --------------------------------------
+ * Replaces running a Vary sweep's calculations one after the other
+ * (Vary.cpp:3-80, SF_Box.cpp:145-176: one SF_System + SF_Solve per `start`
+ * block). Every problem is the calculation
+ *   lat : Lat1DFlat1stO, n_layers, lowerbound surface (frozen wall W in
+ *         layer 0), upperbound mirror1, lambda;
+ *   mol A_N free at phibulk; mol S monomer solvent;
+ *   chi A-S = chi_as, A-W = chi_s, S-W = 0.
+ * The iteration vector of a problem has two values per layer z = 1..n in
+ * layer order (A, then S), as sfx_residual lays out the same system; the
+ * problems are concatenated in order (offsets from sfx_batch_sizes). */
+typedef struct sfx_adsorption_desc {
+  int n_layers;    /* 2..1024 */
+  int N;           /* homopolymer length, >= 2 */
+  double lambda;   /* step weight, (0, 1/2] */
+  double chi_as;   /* polymer-solvent */
+  double chi_s;    /* polymer-wall */
+  double phibulk;  /* homopolymer bulk volume fraction, (0, 1) */
+} sfx_adsorption_desc;
+
+typedef struct sfx_batch sfx_batch;
+/* opt may be NULL; only SFX_FP64 is accepted. */
+int sfx_batch_create(const sfx_adsorption_desc* problems, int count, const sfx_options* opt,
+                     sfx_batch** out);
+int sfx_batch_destroy(sfx_batch* b);
+/* Total iteration variables; offsets (count+1 entries, may be NULL) of every
+ * problem's part of the iteration vector. */
+int sfx_batch_sizes(const sfx_batch* b, long long* nvar, long long* offsets);
+/* Threads per problem the tuner chose at creation and the residual time (ms)
+ * it measured at 64, 128 and 256 threads (ms may be NULL). */
+int sfx_batch_tuning(const sfx_batch* b, int* threads, float* ms);
+/* Residuals of every problem at x (host or device per `where`). */
+int sfx_batch_residual(sfx_batch* b, const double* x, double* f, int where);
+/* DIIS (MPS.cpp:228-284) on every problem at once from x (in/out). Per
+ * problem: evaluations after the first (iterations) and the final residual
+ * norm (host arrays of count, may be NULL). A converged problem returns the
+ * point whose residual was measured. SFX_E_NOTCONVERGED if any problem did
+ * not converge; x and the arrays are filled in either case. */
+int sfx_batch_solve(sfx_batch* b, const sfx_solver_opts* o, double* x, int where, int* iterations,
+                    double* norms, sfx_solve_stats* stats);
+/* ln G_N of every problem's homopolymer at the last evaluation (count values). */
+int sfx_batch_get_lngn(const sfx_batch* b, double* out);
+/* phi of one problem at the last evaluation: state 0 = A, 1 = S; n_layers+2
+ * values, halo layers 0. */
+int sfx_batch_get_phi(const sfx_batch* b, int problem, int state, double* out);
+
 /* ---- per-rank lattice primitives (Cuda_tools.h:3-26 / Lat1stO.h) -------- */
 /* All arrays are device pointers of M doubles; stream may be NULL. */
 typedef struct sfx_lattice sfx_lattice;

@@ -282,6 +282,9 @@ int orc_mps_scf(orc_mps* p, double* x, double tol, int max_iter, int m, double e
   double* Ci = calloc(m, sizeof(double));
   double* g = calloc(iv, sizeof(double));
   int it = 0, k_diis = 1, k = 0;
+  /* MPS.cpp:263 starts from x = x0 = 0; from another start x0 is that start,
+   * so x_x0[0] is the first step in both cases */
+  memcpy(x0, x, iv * sizeof(double));
   orc_mps_compute_g(p, x, g);
   for (long i = 0; i < iv; i++) x[i] += -eta * g[i];
   for (long i = 0; i < iv; i++) x_x0[i] = x[i] - x0[i];

@@ -554,6 +554,7 @@ int orc_sys_solve_diis(orc_sys* s, double* x, double tol, int max_iter, int m, d
   double* Ci = calloc(m, sizeof(double));
   double* g = calloc(iv, sizeof(double));
   int it = 0, k_diis = 1, k = 0;
+  memcpy(x0, x, iv * sizeof(double)); /* the start (0 in MPS.cpp:263) */
   orc_sys_residual(s, x, g);
   for (long i = 0; i < iv; i++) x[i] += -eta * g[i];
   for (long i = 0; i < iv; i++) x_x0[i] = x[i] - x0[i];

@@ -113,3 +113,20 @@ NAMES = ["1d_flat_adsorption", "1d_flat_odd_mirror2", "1d_cylinder", "1d_sphere"
 
 def start(lat, sysd, seed=11, sigma=0.2):
     return np.random.default_rng(seed).normal(0.0, sigma, sysd.nvar(lat))
+
+
+def store_ranks(lat, sysd):
+    """Forward ranks the backward pass needs for the largest molecule: N for a
+    second-order or asymmetric chain, N/2 for a palindrome, 0 for a monomer
+    (SF_Homopol1stO.cpp:555, SF_Copol1stO.cpp:49-57)."""
+    best = 0
+    for seq in sysd.sequences():
+        n = len(seq)
+        if n == 1:
+            r = 0
+        elif lat.order == 2 or seq != seq[::-1]:
+            r = n
+        else:
+            r = n // 2
+        best = max(best, r)
+    return best

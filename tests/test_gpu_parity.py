@@ -68,20 +68,67 @@ def test_fp32_logscaled_against_fp64_oracle(built, name):
 
 
 def test_fp32_scaling_survives_underflow(sfx, orc):
-    """A long chain on a strongly repulsive field: G_N ~ e^-N*u drops below the
-    FP32 range (1e-38) without the per-rank log-scale; with it, ln G_N and phi
-    stay accurate."""
+    """A long free chain on a repulsive field: G_N ~ e^{-N u} drops below the
+    FP32 range (1e-38) without the per-rank log-scale; with it, ln G_N and the
+    residual stay accurate."""
     from field_amd.spec import LatticeSpec, MoleculeSpec, SystemSpec
     lat = LatticeSpec(1, (40,), lowerbound=("mirror1",), upperbound=("mirror1",), lambda_=1 / 6)
     sysd = SystemSpec(("A", "S"), np.array([[0, 0.5], [0.5, 0]]),
-                      [MoleculeSpec("p", "(A)300", "restricted", theta=4.0), MoleculeSpec("s", "(S)1", "solvent")])
+                      [MoleculeSpec("p", "(A)300", "free", phibulk=0.01), MoleculeSpec("s", "(S)1", "solvent")])
     G, O = sfx.System(lat, sysd), orc.System(lat, sysd)
-    x = np.full(G.nvar, 0.6) + np.random.default_rng(3).normal(0, 0.1, G.nvar)
+    rng = np.random.default_rng(3)
+    x = rng.normal(0, 0.1, G.nvar)
+    x[0::2] += 0.6  # u_A (iteration variables alternate A, S per site)
     f32 = G.residual(x, precision="fp32")
     fo = O.residual(x)
     assert O.lngn(0) < math.log(1e-38)  # the unscaled FP32 propagator would underflow
-    assert abs(G.lngn(0) - O.lngn(0)) < 1e-4 * abs(O.lngn(0))
-    assert np.abs(f32 - fo).max() < 1e-3
+    assert abs(G.lngn(0) - O.lngn(0)) < 1e-5 * abs(O.lngn(0))
+    err = np.abs(f32 - fo).max()
+    print(f"ln G_N = {O.lngn(0):.2f}; FP32 residual error {err:.2e}")
+    assert err < 1e-4
+
+
+def test_negative_solvent_bulk_fraction(sfx, orc):
+    """A restricted chain whose theta / G_N exceeds 1 leaves the solvent a
+    negative bulk fraction (the closure 1 - sum phi_b); the reference
+    multiplies by it as it is, and so must both precisions."""
+    from field_amd.spec import LatticeSpec, MoleculeSpec, SystemSpec
+    lat = LatticeSpec(1, (40,), lowerbound=("mirror1",), upperbound=("mirror1",), lambda_=1 / 6)
+    sysd = SystemSpec(("A", "S"), np.array([[0, 0.5], [0.5, 0]]),
+                      [MoleculeSpec("p", "(A)20", "restricted", theta=48.0), MoleculeSpec("s", "(S)1", "solvent")])
+    G, O = sfx.System(lat, sysd), orc.System(lat, sysd)
+    x = np.random.default_rng(4).normal(0, 0.02, G.nvar)
+    fo = O.residual(x)
+    assert O.phibulk(1) < 0
+    f64 = G.residual(x, precision="fp64")
+    assert abs(G.phibulk(1) - O.phibulk(1)) < 1e-12
+    np.testing.assert_allclose(f64, fo, rtol=0, atol=1e-10 * max(1, np.abs(fo).max()))
+    f32 = G.residual(x, precision="fp32")
+    assert np.abs(f32 - fo).max() < 2e-4
+    v = np.random.default_rng(5).normal(0, 1, G.nvar)
+    h = 1e-6
+    fd = (O.residual(x + h * v) - O.residual(x - h * v)) / (2 * h)
+    jv = G.jvp(x, v, precision="fp64")
+    assert np.abs(jv - fd).max() / np.abs(fd).max() < 1e-6
+
+
+@pytest.mark.parametrize("name", ["1d_flat_adsorption", "2d_cylinder", "3d_sc_melt", "1d_sphere",
+                                  "3d_sc_order2_homopolymer", "3d_sc_order2_copolymer"])
+def test_checkpointed_sweeps_match_full_store(built, name, sfx):
+    """A memory budget below the full propagator store switches to sqrt(N)
+    checkpoints with recomputation; residuals and JVPs are bit-identical to the
+    full store in both precisions."""
+    lat, sysd, G, O = built[name]
+    comps = 6 if lat.order == 2 else 1
+    full = cases.store_ranks(lat, sysd) * comps * G.M * 8
+    C = sfx.System(lat, sysd, memory_budget=full - 1)
+    x = cases.start(lat, sysd, seed=21, sigma=0.1)
+    v = np.random.default_rng(22).normal(0, 1, G.nvar)
+    for prec in ("fp64", "fp32"):
+        np.testing.assert_array_equal(C.residual(x, precision=prec), G.residual(x, precision=prec))
+        np.testing.assert_array_equal(C.jvp(x, v, precision=prec), G.jvp(x, v, precision=prec))
+    with pytest.raises(sfx.SFError):
+        sfx.System(lat, sysd, memory_budget=2 * comps * G.M * 8)
 
 
 @pytest.mark.parametrize("name", cases.NAMES)

@@ -379,3 +379,18 @@ def test_order2_uniform_field_closed_form(orc):
     S.compute_phi(np.full(S.nvar, 0.25))
     assert abs(S.lngn(0) - (math.log(60) - 9 * 0.25)) < 1e-12
     np.testing.assert_allclose(S.phi(0)[lat.interior_mask()], 12.0 / 60, rtol=1e-12)
+
+
+def test_torch_melt_reference_matches_oracle(orc):
+    """The differentiable torch restatement used as the reference gradient
+    reproduces the oracle's melt residual."""
+    import torch
+
+    import torch_ref
+
+    n, na, nb = 5, 3, 4
+    lat, sysd, x = configs.melt_3d(n=n, n_a=na, n_b=nb, chi=0.4, sigma=0.2, seed=9)
+    fo = orc.System(lat, sysd).residual(x)
+    u = torch_ref.to_interior(torch.from_numpy(x), n)
+    f = torch_ref.melt_residual(u, [0] * na + [1] * nb, [[0, 0.4], [0.4, 0]], float(n ** 3), (0.0, 0.0))
+    np.testing.assert_allclose(torch_ref.to_padded(f, n).numpy(), fo, rtol=0, atol=1e-12)
