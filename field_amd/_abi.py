@@ -80,6 +80,7 @@ class SolverOpts(C.Structure):
         ("gmres_rtol", C.c_double),
         ("jvp_precision", C.c_int),
         ("fp64_switch", C.c_double),
+        ("precondition", C.c_int),
         ("verbose", C.c_int),
     ]
 

@@ -180,8 +180,23 @@ int sfx_solver_defaults(sfx_solver_opts* o) {
   o->gmres_rtol = 1e-3;
   o->jvp_precision = SFX_FP32;
   o->fp64_switch = 1e-3;
+  o->precondition = 1;
   o->verbose = 0;
   return SFX_OK;
+}
+
+int sfx_precondition(sfx_system* s, const double* r, double* out, int where) {
+  return guarded([&] {
+    SFX_REQUIRE(s && r && out, SFX_E_INVALID, "NULL argument");
+    System& S = s->s;
+    SFX_REQUIRE(precond_supported(S), SFX_E_UNSUPPORTED,
+                "the Fourier-space preconditioner needs a periodic simple-cubic melt");
+    if (!S.precond) S.precond = precond_create(S);
+    DevBuf<double> a((size_t)S.nvar), b((size_t)S.nvar);
+    load_vec(S, r, where, a.p);
+    precond_apply(S.precond, S, a.p, b.p);
+    S.store_f(b.p, out, where);
+  });
 }
 
 int sfx_solve(sfx_system* s, const sfx_solver_opts* o, double* x, int where, sfx_solve_stats* stats) {

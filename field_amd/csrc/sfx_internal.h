@@ -75,6 +75,7 @@ struct DevLattice {
   int surf1[3], surf2[3];
   int mirror2_1[3], mirror2_2[3];
   int periodic[3];
+  int flip1[3], flip2[3];  // 2nd order: a halo read reverses the axis' components
   int order;
   double l1, l1chi, ladj;
   double l0_2d, l1x, l1y;

@@ -127,11 +127,19 @@ void lattice_init(Lattice& L, const sfx_lattice_desc& d) {
     D.mirror2_1[k] = lo == SFX_BOUND_MIRROR2;
     D.mirror2_2[k] = up == SFX_BOUND_MIRROR2;
     D.periodic[k] = lo == SFX_BOUND_PERIODIC;
+    // second-order halos (SetBx1(P,Q) / SetBxm(P,Q) and their y, z forms):
+    // 3-D swaps the +/- components at every non-periodic bound
+    // (Lat3D.cpp:1276-1290, 1428-1440); 1-D and 2-D swap at the lower bound
+    // only for surface and mirror1 (`bound1 > 2` copies plainly,
+    // Lat1DFlat.cpp:658-666, Lat2DFlat.cpp:1343-1393) and at the upper bound
+    // for every non-periodic choice (Lat1DFlat.cpp:673-682, Lat2DFlat.cpp:1363-1410)
+    D.flip1[k] = D.grad == 3 ? lo != SFX_BOUND_PERIODIC : (lo == SFX_BOUND_SURFACE || lo == SFX_BOUND_MIRROR1);
+    D.flip2[k] = up != SFX_BOUND_PERIODIC;
   }
   D.order = d.order == 2 ? 2 : 1;
   if (D.order == 2)
-    SFX_REQUIRE(D.grad == 3 && d.lattice_type == SFX_LAT_STANDARD, SFX_E_UNSUPPORTED,
-                "second-order chains are built for the 3-D simple-cubic lattice");
+    SFX_REQUIRE(d.lattice_type == SFX_LAT_STANDARD && d.geometry == SFX_FLAT, SFX_E_UNSUPPORTED,
+                "second-order chains are built for flat standard lattices (Lat1D2ndO, Lat2D2ndO, Lat3D2ndO)");
   D.ladj = d.layer_adjustment > 0 ? d.layer_adjustment : 1.0;
   D.l1 = d.lambda;
   D.l1chi = d.lambda_chi > 0 ? d.lambda_chi : d.lambda;

@@ -6,6 +6,9 @@
 // ||f|| is above fp64_switch and with FP64 sweeps below it; convergence is
 // declared on the FP64 residual only, so the FP32 Jacobian only steers the
 // step (iterative refinement). Backtracking on ||f|| globalises the step.
+// Where it applies, GMRES is right-preconditioned with the Fourier-space
+// Jacobian of the homogeneous melt (sfx_precond.hip): it solves (J P) z = -f
+// and steps dx = P z.
 #include <cmath>
 
 #include "sfx_solve.h"
@@ -19,8 +22,23 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
   const int m = std::max(1, std::min(o.gmres_restart, 30));
   VecOps V;
   V.init(n, S.stream);
-  DevBuf<double> basis((size_t)(m + 1) * n), w(n), dx(n), x0(n), r(n);
+  DevBuf<double> basis((size_t)(m + 1) * n), w(n), dx(n), x0(n), r(n), pz(n);
   double* x = S.d_u;
+  Precond* pc = nullptr;
+  if (o.precondition && precond_supported(S)) {
+    if (!S.precond) S.precond = precond_create(S);
+    pc = S.precond;
+  }
+  // the GMRES operator v -> J P v (J v without a preconditioner)
+  auto op = [&](const double* v, double* out) {
+    if (pc) {
+      precond_apply(pc, S, v, pz.p);
+      jvp_at_base(S, pz.p, out, o.jvp_precision);
+    } else {
+      jvp_at_base(S, v, out, o.jvp_precision);
+    }
+    st.jvp_evals++;
+  };
   auto Vk = [&](int k) { return basis.p + (size_t)k * n; };
   hipEvent_t e0, e1;
   SFX_HIP(hipEventCreate(&e0));
@@ -65,9 +83,8 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
         V.axpby(-1.0 / rn, r.p, 0.0, Vk(0));
         beta = rn;
       } else {
-        jvp_at_base(S, dx.p, w.p, o.jvp_precision);
-        st.jvp_evals++;
-        V.axpby(-1.0, r.p, -1.0, w.p);  // w = -r - J dx
+        op(dx.p, w.p);
+        V.axpby(-1.0, r.p, -1.0, w.p);  // w = -r - J P z
         beta = std::sqrt(V.dot(w.p, w.p));
         if (beta <= eta * beta0) break;
         V.axpby(1.0 / beta, w.p, 0.0, Vk(0));
@@ -77,8 +94,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
       int j = 0;
       double res = beta;
       for (; j < m && total < o.gmres_max_iter; j++) {
-        jvp_at_base(S, Vk(j), w.p, o.jvp_precision);
-        st.jvp_evals++;
+        op(Vk(j), w.p);
         total++;
         // classical Gram-Schmidt, twice
         std::vector<double> h(j + 1, 0.0);
@@ -151,14 +167,22 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
       if (o.verbose) std::fprintf(stderr, "  gmres cycle=%d its=%d est=%.3e\n", cycle, j, res / beta0);
       if (res <= eta * beta0) break;
     }
+    if (pc) {  // dx = P z
+      V.copy(dx.p, w.p);
+      precond_apply(pc, S, w.p, dx.p);
+    }
     // ---- backtracking line search on ||f|| ----
     V.copy(x, x0.p);
-    double alpha = 1.0, rt = 0;
+    double alpha = 1.0, rt = 0, best = INFINITY, best_alpha = 0;
     bool accepted = false;
     for (int ls = 0; ls < 12; ls++) {
       V.copy(x0.p, x);
       V.axpby(alpha, dx.p, 1.0, x);
       rt = eval(prec);
+      if (std::isfinite(rt) && rt < best) {
+        best = rt;
+        best_alpha = alpha;
+      }
       if (std::isfinite(rt) && rt <= (1 - 1e-4 * alpha) * rn) {
         accepted = true;
         break;
@@ -166,8 +190,18 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
       alpha *= 0.5;
     }
     if (!accepted) {
-      // keep the smallest step; the next Newton iteration re-linearises there
-      if (o.verbose) std::fprintf(stderr, "  line search failed, |f|=%.3e\n", rt);
+      // take the best finite trial if it lowers ||f|| at all, else stop at x0
+      V.copy(x0.p, x);
+      if (best < rn) {
+        V.axpby(best_alpha, dx.p, 1.0, x);
+        rt = eval(prec);
+      } else {
+        rt = eval(prec);
+        if (o.verbose) std::fprintf(stderr, "  line search found no descent, |f|=%.3e\n", rt);
+        rn = rt;
+        break;
+      }
+      if (o.verbose) std::fprintf(stderr, "  line search fell back to alpha=%g, |f|=%.3e\n", best_alpha, rt);
     }
     rn = rt;
   }

@@ -1,5 +1,6 @@
-// sfx_order2.hip -- second-order (semiflexible) propagators on the 3-D simple
-// cubic lattice (Lat3D2ndO.cpp), base and tangent.
+// sfx_order2.hip -- second-order (semiflexible) propagators on flat standard
+// lattices (Lat3D2ndO.cpp; Lat2D2ndO.cpp and Lat1D2ndO.cpp as the same six
+// components with the missing axes homogeneous), base and tangent.
 //
 // A propagator has six direction components stored [k][M] per rank; the bond
 // into a site points along e_k, e = {+x, +y, +z, -z, -y, -x}, so 5-k is the
@@ -23,18 +24,22 @@ __device__ __forceinline__ float wmax2(float v) {
 }
 
 // neighbour (x,y,z) + d along axis a (d = +-1): returns the site index and
-// whether a mirror was crossed (component reversal)
+// whether the halo read reverses the axis' components. An axis beyond the
+// lattice's gradients is homogeneous: the neighbour is the site itself (the
+// 1-D and 2-D lattices lump those directions, Lat1D2ndO.cpp:32-62,
+// Lat2D2ndO.cpp:31-92).
 __device__ __forceinline__ long long nb2(const DevLattice& L, int x, int y, int z, int a, int d, bool& flip) {
   int c[3] = {x, y, z};
   int n[3] = {L.nx, L.ny, L.nz};
-  int v = c[a] + d;
   flip = false;
+  if (a >= L.grad) return sidx(L, x, y, z);
+  int v = c[a] + d;
   if (v == 0) {
     v = L.b1[a];
-    flip = !L.periodic[a];
+    flip = L.flip1[a];
   } else if (v == n[a] + 1) {
     v = L.b2[a];
-    flip = !L.periodic[a];
+    flip = L.flip2[a];
   }
   c[a] = v;
   return sidx(L, c[0], c[1], c[2]);
@@ -236,7 +241,9 @@ __global__ void k_wsum6(DevLattice L, const T* __restrict__ G, const T* __restri
     int x, y, z;
     interior_coords(L, q, x, y, z);
     long long r = sidx(L, x, y, z);
-    double w = lngn_weight(L, x, y, z);
+    // 1-D sums every layer plainly (Lat1D2ndO.cpp ComputeLnGN); 2-D and 3-D
+    // halve mirror2 edges (Lat2D2ndO.cpp:387-439, Lat3D2ndO.cpp:996-1025)
+    double w = L.grad == 1 ? 1.0 : lngn_weight(L, x, y, z);
     for (int k = 0; k < 6; k++) {
       a += (double)G[(long long)k * L.M + r] * w;
       if (dG) b += (double)dG[(long long)k * L.M + r] * w;

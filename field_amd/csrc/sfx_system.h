@@ -99,6 +99,7 @@ struct Profiler {
 
 struct Tangent;  // JVP / VJP buffers (sfx_adjoint.hip)
 void tangent_free(Tangent* t);
+struct Precond;
 
 struct System {
   Lattice lat;
@@ -147,6 +148,7 @@ struct System {
   double* d_xabi = nullptr;     // nvar: ABI-layout staging
   int npart = 0;
   Tangent* tan = nullptr;
+  Precond* precond = nullptr;  // Newton-Krylov preconditioner, built on first use
   Profiler prof;
   bool phi_valid = false;
   int last_precision = SFX_FP64;
@@ -167,5 +169,11 @@ struct System {
   template <typename T, typename A>
   void sweep(int m, bool scaled);
 };
+
+// Fourier-space Newton-Krylov preconditioner (sfx_precond.hip)
+bool precond_supported(const System& S);
+Precond* precond_create(const System& S);
+void precond_apply(Precond* p, const System& S, const double* r, double* out);
+void precond_free(Precond* p);
 
 }  // namespace sfx

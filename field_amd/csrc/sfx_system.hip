@@ -66,6 +66,7 @@ System::~System() {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   tangent_free(tan);
+  precond_free(precond);
   if (stream) (void)hipStreamDestroy(stream);
 }
 

@@ -39,6 +39,7 @@ _SIGS = {
     "sfx_free_energy": (C.c_int, [C.c_void_p, PD]),
     "sfx_solver_defaults": (C.c_int, [C.c_void_p]),
     "sfx_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "sfx_precondition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "sfx_profile_start": (C.c_int, [C.c_void_p]),
     "sfx_profile_stop": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sfx_batch_create": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -164,6 +165,14 @@ class System:
         px, where = _ptr(x)
         po, _ = _ptr(out)
         check(lib().sfx_residual(self.h, px, po, where, self._p(precision)))
+        return out
+
+    def precondition(self, r):
+        """The Newton-Krylov preconditioner applied to r (sfx_precondition)."""
+        r = self._vec(r)
+        out = np.zeros(self.nvar) if isinstance(r, np.ndarray) else r.new_zeros(self.nvar)
+        pr, where = _ptr(r)
+        check(lib().sfx_precondition(self.h, pr, _ptr(out)[0], where))
         return out
 
     def compute_phi(self, x, precision=None):

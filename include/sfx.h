@@ -170,6 +170,9 @@ typedef struct sfx_solver_opts {
   double gmres_rtol;      /* NK: relative Krylov tolerance (forcing term) */
   int jvp_precision;      /* NK: precision of the tangent sweeps */
   double fp64_switch;     /* NK: ||f|| below which residual sweeps run in FP64 */
+  int precondition;       /* NK: 1 = right-precondition GMRES with the Fourier-space Jacobian of the
+                             homogeneous melt where it applies (periodic simple-cubic melt form);
+                             0 = none. No counterpart in the reference (SFNewton has none). */
   int verbose;
 } sfx_solver_opts;
 
@@ -184,6 +187,10 @@ typedef struct sfx_solve_stats {
 } sfx_solve_stats;
 
 int sfx_solver_defaults(sfx_solver_opts* o);
+/* Applies the NK preconditioner (see sfx_solver_opts.precondition) to r,
+ * both in the iteration-vector layout; SFX_E_UNSUPPORTED where it does not
+ * apply. A diagnostic with no counterpart in the reference. */
+int sfx_precondition(sfx_system* s, const double* r, double* out, int where);
 /* Solves f(x)=0 from x (in/out, host or device per `where`). */
 int sfx_solve(sfx_system* s, const sfx_solver_opts* o, double* x, int where,
               sfx_solve_stats* stats);

@@ -38,6 +38,8 @@ double orc_lngn(const orc_lat* L, const double* G);
 void orc_side(const orc_lat* L, double* P, double* out);
 /* second order (3-D simple cubic), six direction components per site */
 void orc_setb6(const orc_lat* L, double* P);
+int orc_ncomp2(const orc_lat* L);
+double orc_wcomp2(const orc_lat* L, int k);
 void orc_propagate2(const orc_lat* L, double* Gin, const double* g, double* Gout, double S, int backward);
 double orc_lngn2(const orc_lat* L, const double* G);
 

@@ -1,6 +1,7 @@
 /*
  * orc_lat2.c -- ORACLE (test infrastructure only): restatement of the
- * second-order (semiflexible) simple-cubic lattice, Lat3D2ndO.cpp.
+ * second-order (semiflexible) flat lattices: Lat3D2ndO.cpp (six components),
+ * Lat2D2ndO.cpp (five) and Lat1D2ndO.cpp (three), each in its own lumped form.
  *
  * A propagator has six direction components k (k*M + site): the bond into the
  * site points along e_k with e = {+x, +y, +z, -z, -y, -x}, so component 5-k is
@@ -65,7 +66,149 @@ static double T(int k, int j, double S) {
   return (1.0 - S) / 4.0;
 }
 
+/* ---- 1-D and 2-D forms ----------------------------------------------------
+ * Lat1D2ndO: components {+x, in-layer (each of the four), -x}.
+ * Lat2D2ndO: components {+x, +y, perpendicular (each of the two), -y, -x}.
+ * Halo rules of the (P,Q) setters: the lower bound swaps the pair for surface
+ * and mirror1 and copies plainly otherwise (`bound1 > 2`, Lat1DFlat.cpp:658-666,
+ * Lat2DFlat.cpp:1343-1355, 1382-1393); the upper bound copies plainly only
+ * when periodic (`bound2 < numLayers-1`, Lat1DFlat.cpp:673-682,
+ * Lat2DFlat.cpp:1363-1375, 1401-1410). Copies are sequential, as there. */
+static int lower_swaps(const orc_lat* L, int a) {
+  return L->code1[a] == SFX_BOUND_SURFACE || L->code1[a] == SFX_BOUND_MIRROR1;
+}
+static int upper_swaps(const orc_lat* L, int a) { return L->code2[a] != SFX_BOUND_PERIODIC; }
+
+/* x-halo rows (all y) from row b; swap pair (p, q) or plain */
+static void rows(const orc_lat* L, double* p, double* q, int dst, int src, int swap) {
+  long jx = L->MY;
+  for (int y = 0; y < L->MY; y++) {
+    if (swap && q) {
+      p[dst * jx + y] = q[src * jx + y];
+      q[dst * jx + y] = p[src * jx + y];
+    } else {
+      p[dst * jx + y] = p[src * jx + y];
+      if (q) q[dst * jx + y] = q[src * jx + y];
+    }
+  }
+}
+static void cols(const orc_lat* L, double* p, double* q, int dst, int src, int swap) {
+  long jx = L->MY;
+  for (int x = 0; x < L->MX; x++) {
+    if (swap && q) {
+      p[x * jx + dst] = q[x * jx + src];
+      q[x * jx + dst] = p[x * jx + src];
+    } else {
+      p[x * jx + dst] = p[x * jx + src];
+      if (q) q[x * jx + dst] = q[x * jx + src];
+    }
+  }
+}
+
+static void setb3(const orc_lat* L, double* P) {
+  long M = L->M;
+  double *p0 = P, *p1 = P + M, *p2 = P + 2 * M;
+  int top = L->MX - 1;
+  /* SetBx1(gz0,gz2); SetBx1(gz1); SetBxm(gz0,gz2); SetBxm(gz1) */
+  rows(L, p0, p2, 0, L->b1[0], lower_swaps(L, 0));
+  rows(L, p1, NULL, 0, L->b1[0], 0);
+  rows(L, p0, p2, top, L->b2[0], upper_swaps(L, 0));
+  rows(L, p1, NULL, top, L->b2[0], 0);
+}
+
+static void setb5(const orc_lat* L, double* P) {
+  long M = L->M;
+  double* p[5];
+  for (int k = 0; k < 5; k++) p[k] = P + k * M;
+  int tx = L->MX - 1, ty = L->MY - 1;
+  /* Lat2D2ndO.cpp:47-50: x lower, y lower, x upper, y upper */
+  rows(L, p[0], p[4], 0, L->b1[0], lower_swaps(L, 0));
+  for (int k = 1; k <= 3; k++) rows(L, p[k], NULL, 0, L->b1[0], 0);
+  cols(L, p[1], p[3], 0, L->b1[1], lower_swaps(L, 1));
+  cols(L, p[0], NULL, 0, L->b1[1], 0);
+  cols(L, p[2], NULL, 0, L->b1[1], 0);
+  cols(L, p[4], NULL, 0, L->b1[1], 0);
+  rows(L, p[0], p[4], tx, L->b2[0], upper_swaps(L, 0));
+  for (int k = 1; k <= 3; k++) rows(L, p[k], NULL, tx, L->b2[0], 0);
+  cols(L, p[1], p[3], ty, L->b2[1], upper_swaps(L, 1));
+  cols(L, p[0], NULL, ty, L->b2[1], 0);
+  cols(L, p[2], NULL, ty, L->b2[1], 0);
+  cols(L, p[4], NULL, ty, L->b2[1], 0);
+}
+
+/* Lat1D2ndO.cpp:32-62 (forward) and :117-145 (backward) */
+static void propagate_1d(const orc_lat* L, double* Gin, const double* g, double* Gout, double S, int backward) {
+  long M = L->M;
+  double Lw = (1.0 - S) / 4.0;
+  setb3(L, Gin);
+  const double *a0 = Gin, *a1 = Gin + M, *a2 = Gin + 2 * M;
+  for (long z = 1; z <= L->n[0]; z++) {
+    double o0, o1, o2;
+    if (!backward) {
+      o0 = S * a0[z - 1] + 4 * Lw * a1[z - 1];
+      o1 = Lw * a0[z] + (2 * Lw + S) * a1[z] + Lw * a2[z];
+      o2 = 4 * Lw * a1[z + 1] + S * a2[z + 1];
+    } else {
+      o0 = S * a0[z + 1] + 4 * Lw * a1[z];
+      o1 = Lw * a0[z + 1] + (2 * Lw + S) * a1[z] + Lw * a2[z - 1];
+      o2 = S * a2[z - 1] + 4 * Lw * a1[z];
+    }
+    Gout[z] = o0 * g[z];
+    Gout[M + z] = o1 * g[z];
+    Gout[2 * M + z] = o2 * g[z];
+  }
+  setb3(L, Gout);
+}
+
+/* Lat2D2ndO.cpp:31-92 (forward) and :167-219 (backward) */
+static void propagate_2d(const orc_lat* L, double* Gin, const double* g, double* Gout, double S, int backward) {
+  long M = L->M, jx = L->MY;
+  double Lw = (1.0 - S) / 4.0;
+  setb5(L, Gin);
+  const double* a[5];
+  for (int k = 0; k < 5; k++) a[k] = Gin + k * M;
+  for (int x = 1; x <= L->n[0]; x++)
+    for (int y = 1; y <= L->n[1]; y++) {
+      long i = x * jx + y;
+      double o[5];
+      if (!backward) {
+        long p = i - jx, q = i - 1, r = i + 1, u = i + jx;
+        o[0] = S * a[0][p] + Lw * a[1][p] + 2 * Lw * a[2][p] + Lw * a[3][p];
+        o[1] = Lw * a[0][q] + S * a[1][q] + 2 * Lw * a[2][q] + Lw * a[4][q];
+        o[2] = Lw * a[0][i] + Lw * a[1][i] + S * a[2][i] + Lw * a[3][i] + Lw * a[4][i];
+        o[3] = Lw * a[0][r] + 2 * Lw * a[2][r] + S * a[3][r] + Lw * a[4][r];
+        o[4] = Lw * a[1][u] + 2 * Lw * a[2][u] + Lw * a[3][u] + S * a[4][u];
+      } else {
+        o[0] = S * a[0][i + jx] + Lw * a[1][i + 1] + 2 * Lw * a[2][i] + Lw * a[3][i - 1];
+        o[1] = Lw * a[0][i + jx] + S * a[1][i + 1] + 2 * Lw * a[2][i] + Lw * a[4][i - jx];
+        o[2] = Lw * a[0][i + jx] + Lw * a[1][i + 1] + S * a[2][i] + Lw * a[3][i - 1] + Lw * a[4][i - jx];
+        o[3] = Lw * a[0][i + jx] + 2 * Lw * a[2][i] + S * a[3][i - 1] + Lw * a[4][i - jx];
+        o[4] = Lw * a[1][i + 1] + 2 * Lw * a[2][i] + Lw * a[3][i - 1] + S * a[4][i - jx];
+      }
+      for (int k = 0; k < 5; k++) Gout[k * M + i] = o[k] * g[i];
+    }
+  setb5(L, Gout);
+}
+
+int orc_ncomp2(const orc_lat* L) { return L->grad == 1 ? 3 : L->grad == 2 ? 5 : 6; }
+
+/* multiplicity of a lumped component in ConnectG and ComputeLnGN
+ * (Lat1D2ndO.cpp ConnectG: 1, 4, 1; Lat2D2ndO.cpp:270-282: 1, 1, 2, 1, 1) */
+double orc_wcomp2(const orc_lat* L, int k) {
+  if (L->grad == 1) return k == 1 ? 4.0 : 1.0;
+  if (L->grad == 2) return k == 2 ? 2.0 : 1.0;
+  return 1.0;
+}
+
 void orc_propagate2(const orc_lat* L, double* Gin, const double* g, double* Gout, double S, int backward) {
+  if (L->grad == 1) {
+    propagate_1d(L, Gin, g, Gout, S, backward);
+    return;
+  }
+  if (L->grad == 2) {
+    propagate_2d(L, Gin, g, Gout, S, backward);
+    return;
+  }
   long M = L->M, jx = L->jx, jy = L->jy;
   long off[6] = {jx, jy, 1, -1, -jy, -jx}; /* e_k as a site offset */
   orc_setb6(L, Gin);
@@ -89,6 +232,26 @@ void orc_propagate2(const orc_lat* L, double* Gin, const double* g, double* Gout
 
 double orc_lngn2(const orc_lat* L, const double* G) {
   double value = 0;
+  if (L->grad == 1) { /* plain sum over the layers (Lat1D2ndO.cpp ComputeLnGN) */
+    for (int z = 1; z <= L->n[0]; z++)
+      for (int k = 0; k < 3; k++) value += orc_wcomp2(L, k) * G[k * L->M + z];
+    return log(value / 6.0);
+  }
+  if (L->grad == 2) { /* Lat2D2ndO.cpp:387-439: mirror2 edges count half */
+    for (int x = 1; x <= L->n[0]; x++)
+      for (int y = 1; y <= L->n[1]; y++) {
+        long i = x * (long)L->MY + y;
+        double w = 1.0;
+        if (x == 1 && L->code1[0] == SFX_BOUND_MIRROR2) w *= 0.5;
+        if (x == L->n[0] && L->code2[0] == SFX_BOUND_MIRROR2) w *= 0.5;
+        if (y == 1 && L->code1[1] == SFX_BOUND_MIRROR2) w *= 0.5;
+        if (y == L->n[1] && L->code2[1] == SFX_BOUND_MIRROR2) w *= 0.5;
+        double s = 0;
+        for (int k = 0; k < 5; k++) s += orc_wcomp2(L, k) * G[k * L->M + i];
+        value += w * s;
+      }
+    return log(value / 6.0);
+  }
   double* c = malloc(L->M * sizeof(double));
   for (int k = 0; k < 6; k++) {
     memcpy(c, G + k * L->M, L->M * sizeof(double));

@@ -233,24 +233,32 @@ static void scale(orc_sys* s, double* phi, double c) {
   for (long z = 0; z < s->M; z++) phi[z] *= c;
 }
 
-/* SF_Homopol2ndO.cpp:152-207 (full sweep, six direction components) */
+/* SF_Homopol2ndO.cpp:152-207 (full sweep over the direction components:
+ * six in 3-D, the lumped five in 2-D and three in 1-D, each weighted by its
+ * multiplicity in the composition) */
 static void mol_compute_phi2(orc_sys* s, omol* o) {
-  long M = s->M, M6 = 6 * s->M;
+  const int nc = orc_ncomp2(s->L);
+  long M = s->M, M6 = nc * s->M;
   int N = o->N;
   double S = s->stiffness;
   double* Gi = s->Gi;
   double* Ginv = s->Ginv;
-  for (int k = 0; k < 6; k++) memcpy(Gi + k * M, s->swf + o->seq[0] * M, sizeof(double) * M);
+  for (int k = 0; k < nc; k++) memcpy(Gi + k * M, s->swf + o->seq[0] * M, sizeof(double) * M);
   for (int r = 2; r <= N; r++)
     orc_propagate2(s->L, Gi + (long)(r - 2) * M6, s->swf + o->seq[r - 1] * M, Gi + (long)(r - 1) * M6, S, 0);
-  for (int k = 0; k < 6; k++) memcpy(Ginv + k * M, s->swf + o->seq[N - 1] * M, sizeof(double) * M);
+  for (int k = 0; k < nc; k++) memcpy(Ginv + k * M, s->swf + o->seq[N - 1] * M, sizeof(double) * M);
   for (int q = N; q >= 1; q--) {
     int t = o->seq[q - 1];
     if (q < N) {
       memcpy(s->Gtmp, Ginv, sizeof(double) * M6);
       orc_propagate2(s->L, s->Gtmp, s->swf + t * M, Ginv, S, 1);
     }
-    for (int k = 0; k < 6; k++) connect(s, o->phi + t * M, Ginv + k * M, Gi + (long)(q - 1) * M6 + k * M);
+    for (int k = 0; k < nc; k++) {
+      const double w = orc_wcomp2(s->L, k);
+      const double* A = Ginv + k * M;
+      const double* B = Gi + (long)(q - 1) * M6 + k * M;
+      for (long z = 0; z < M; z++) o->phi[t * M + z] += w * A[z] * B[z];
+    }
   }
   o->lnGN = orc_lngn2(s->L, Ginv);
   for (int t = 0; t < s->ns; t++) {

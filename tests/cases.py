@@ -103,12 +103,28 @@ def case(name):
                           [MoleculeSpec("p", "(A)3(B)5", "free", phibulk=0.1), MoleculeSpec("s", "(S)1", "solvent")],
                           stiffness=0.3)
         return lat, sysd
+    if name == "1d_order2_adsorption":
+        lat = LatticeSpec(1, (24,), order=2, lowerbound=("surface",), upperbound=("mirror1",), lambda_=1 / 6)
+        st = ("A", "S", "W")
+        sysd = SystemSpec(st, _chi(st, [("A", "S", 0.5), ("A", "W", -1.0)]),
+                          [MoleculeSpec("p", "(A)15", "free", phibulk=0.02), MoleculeSpec("s", "(S)1", "solvent")],
+                          frozen={"W": _wall(lat, "x0")}, stiffness=0.5)
+        return lat, sysd
+    if name == "2d_order2_copolymer":
+        # mirror2 at a lower bound copies components plainly, at an upper bound it swaps them
+        lat = LatticeSpec(2, (7, 9), order=2, lowerbound=("mirror2", "mirror1"), upperbound=("mirror1", "mirror2"),
+                          lambda_=1 / 6)
+        st = ("A", "B", "S")
+        sysd = SystemSpec(st, _chi(st, [("A", "B", 0.8), ("A", "S", 0.2), ("B", "S", 0.6)]),
+                          [MoleculeSpec("p", "(A)4(B)6", "restricted", theta=12.0), MoleculeSpec("s", "(S)1", "solvent")],
+                          stiffness=0.6)
+        return lat, sysd
     raise KeyError(name)
 
 
 NAMES = ["1d_flat_adsorption", "1d_flat_odd_mirror2", "1d_cylinder", "1d_sphere", "2d_flat", "2d_cylinder",
          "3d_sc_melt", "3d_fcc_restricted", "3d_hex_three_molecules", "3d_sc_order2_homopolymer",
-         "3d_sc_order2_copolymer"]
+         "3d_sc_order2_copolymer", "1d_order2_adsorption", "2d_order2_copolymer"]
 
 
 def start(lat, sysd, seed=11, sigma=0.2):

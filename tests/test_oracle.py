@@ -394,3 +394,46 @@ def test_torch_melt_reference_matches_oracle(orc):
     u = torch_ref.to_interior(torch.from_numpy(x), n)
     f = torch_ref.melt_residual(u, [0] * na + [1] * nb, [[0, 0.4], [0.4, 0]], float(n ** 3), (0.0, 0.0))
     np.testing.assert_allclose(torch_ref.to_padded(f, n).numpy(), fo, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("grad,bx", [(1, ("mirror1", "mirror1")), (1, ("periodic", "periodic")),
+                                     (2, ("mirror1", "mirror1")), (2, ("periodic", "periodic"))])
+def test_order2_lumped_forms_match_six_components(orc, grad, bx):
+    """Lat1D2ndO's three and Lat2D2ndO's five lumped components, restated
+    from their own formulas, reproduce the six-component 3-D sweep on the same
+    lattice with the missing axes one periodic layer thick (homogeneous)."""
+    n = (7, 6)[:grad]
+    low, up = (bx[0], "periodic")[:grad], (bx[1], "periodic")[:grad]
+    lat = LatticeSpec(grad, n, order=2, lowerbound=low, upperbound=up, lambda_=1 / 6)
+    lat3 = LatticeSpec(3, tuple(n) + (1,) * (3 - grad), order=2, lowerbound=low + ("periodic",) * (3 - grad),
+                       upperbound=up + ("periodic",) * (3 - grad))
+    sysd = SystemSpec(("A", "B", "S"), np.array([[0, 0.8, 0.3], [0.8, 0, 0.5], [0.3, 0.5, 0]]),
+                      [MoleculeSpec("p", "(A)4(B)5", "restricted", theta=9.0),
+                       MoleculeSpec("q", "(B)3", "free", phibulk=0.1), MoleculeSpec("s", "(S)1", "solvent")],
+                      stiffness=0.45)
+    A, B = orc.System(lat, sysd), orc.System(lat3, sysd)
+    assert A.nvar == B.nvar
+    x = np.random.default_rng(31).normal(0, 0.3, A.nvar)
+    fa, fb = A.residual(x), B.residual(x)
+    np.testing.assert_allclose(fa, fb, rtol=0, atol=1e-12)
+    for m in range(2):
+        assert abs(A.lngn(m) - B.lngn(m)) < 1e-12
+
+
+@pytest.mark.parametrize("grad", [1, 2])
+def test_order2_lumped_mass_balance_and_closed_form(orc, grad):
+    n = (9, 8)[:grad]
+    lat = LatticeSpec(grad, n, order=2, lowerbound=("mirror1",) * grad, upperbound=("mirror1",) * grad,
+                      lambda_=1 / 6)
+    sysd = SystemSpec(("A", "S"), np.array([[0, 0.6], [0.6, 0]]),
+                      [MoleculeSpec("p", "(A)11", "restricted", theta=7.0), MoleculeSpec("s", "(S)1", "solvent")],
+                      stiffness=0.3)
+    S = orc.System(lat, sysd)
+    m = lat.interior_mask()
+    S.compute_phi(np.random.default_rng(5).normal(0, 0.3, S.nvar))
+    assert abs(np.sum(S.phi(0)[m]) - 7.0) < 1e-11
+    # uniform field: every component is g^N, ln G_N = ln(V g^N)
+    S.compute_phi(np.full(S.nvar, 0.2))
+    V = int(np.prod(n))
+    assert abs(S.lngn(0) - (math.log(V) - 11 * 0.2)) < 1e-12
+    np.testing.assert_allclose(S.phi(0)[m], 7.0 / V, rtol=1e-12)
