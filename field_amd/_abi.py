@@ -81,6 +81,7 @@ class SolverOpts(C.Structure):
         ("jvp_precision", C.c_int),
         ("fp64_switch", C.c_double),
         ("precondition", C.c_int),
+        ("ptc_dt0", C.c_double),
         ("verbose", C.c_int),
     ]
 
@@ -114,8 +115,8 @@ KERNEL_NAMES = {K_FWD: "propagate", K_BWD: "propagate+compose", K_TFWD: "tangent
 
 
 class ProfileEntry(C.Structure):
-    _fields_ = [("kind", C.c_int), ("precision", C.c_int), ("launches", C.c_long), ("total_ms", C.c_double),
-                ("bytes_per_launch", C.c_double)]
+    _fields_ = [("kind", C.c_int), ("precision", C.c_int), ("launches", C.c_long), ("ranks", C.c_long),
+                ("total_ms", C.c_double), ("bytes_per_launch", C.c_double)]
 
 
 class ProfileReport(C.Structure):

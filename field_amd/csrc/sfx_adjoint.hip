@@ -14,9 +14,11 @@
 // The transpose product J^T w runs the same recursions backwards (adjoint
 // propagators with the transposed stencil, applied as a scatter).
 #include <cmath>
+#include <type_traits>
 
 #include "sfx_device.h"
 #include "sfx_order2.h"
+#include "sfx_block3.h"
 #include "sfx_solve.h"
 
 namespace sfx {
@@ -88,7 +90,7 @@ __global__ void k_tinit(DevLattice L, const T* __restrict__ g, const T* __restri
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmaxf(m, sm[i]);
-      atomicMax(sc.maxv + sc.step, __float_as_uint(m));
+      put_step_max(sc.maxv, sc.step, m);
       if (blockIdx.x == 0) sc.lnv[sc.step] = 0.0;
     }
   }
@@ -106,7 +108,7 @@ __global__ void __launch_bounds__(kTB) k_tprop(DevLattice L, const T* __restrict
   T scale = T(1), cfac = T(1);
   double lncum = 0;
   if constexpr (SCALED) {
-    float mprev = __uint_as_float(sc.maxv[sc.step - 1]);
+    float mprev = get_step_max(sc.maxv, sc.step - 1);
     if (!(mprev > 0.f)) mprev = 1.f;
     lncum = sc.lnv[sc.step - 1] + log((double)mprev);
     scale = (T)(1.f / mprev);
@@ -134,7 +136,7 @@ __global__ void __launch_bounds__(kTB) k_tprop(DevLattice L, const T* __restrict
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmaxf(m, sm[i]);
-      atomicMax(sc.maxv + sc.step, __float_as_uint(m));
+      put_step_max(sc.maxv, sc.step, m);
       if (blockIdx.x == 0) sc.lnv[sc.step] = lncum;
     }
   }
@@ -226,13 +228,29 @@ __global__ void k_wsum_mono(DevLattice L, const double* __restrict__ g, const do
 // tangent molecule scalars: dmol[0] = dlnGN, [1] = dpref/pref, [2] = dphib
 __global__ void k_tmol_finalize(MolParams mp, const double* p1, const double* p2, int n, const double* molscal,
                                 double* dmol, int nmol) {
-  if (threadIdx.x || blockIdx.x) return;
-  double* d = dmol + mp.index * 4;
+  __shared__ double r1[256 / 64], r2[256 / 64];
   double s1 = 0, s2 = 0;
-  for (int i = 0; i < n; i++) {
-    s1 += p1[i];
-    s2 += p2[i];
+  if (p1 && p2)
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      s1 += p1[i];
+      s2 += p2[i];
+    }
+  for (int o = 32; o > 0; o >>= 1) {
+    s1 += __shfl_xor(s1, o, 64);
+    s2 += __shfl_xor(s2, o, 64);
   }
+  if ((threadIdx.x & 63) == 0) {
+    r1[threadIdx.x >> 6] = s1;
+    r2[threadIdx.x >> 6] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x || blockIdx.x) return;
+  s1 = s2 = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+    s1 += r1[i];
+    s2 += r2[i];
+  }
+  double* d = dmol + mp.index * 4;
   double dln = (p1 && n > 0) ? s2 / s1 : 0.0;
   const double* ms = molscal + mp.index * MS_COUNT;
   double phib = ms[MS_PHIB];
@@ -397,8 +415,9 @@ static __global__ void k_set_ref2(double* scal, const double* lnv, int idx, doub
 // Tangent sweep of molecule m with per-state directions vt (type T, ns*M).
 // The base propagators are recomputed alongside in the same precision; the
 // base phi_m comes from the last compute_phi at the same x.
-template <typename T>
-static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const T* g32or64, const double* v64) {
+// FP32 tangent sweep of a first-order chain on the 3-D simple-cubic lattice,
+// several ranks per launch (sfx_block3.hip); the structure of tsweep below.
+static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const float* g32) {
   const DevLattice& L = S.lat.d;
   const long long Mn = L.M;
   Molecule& mo = S.mols[m];
@@ -406,8 +425,185 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
   double* scal = S.d_molscal + (size_t)m * MS_COUNT;
   double* lnf = S.d_lnf + (size_t)m * (S.maxN + 2);
   double* lnb = S.d_lnb + (size_t)m * (S.maxN + 2);
-  unsigned* maxf = S.d_maxf + (size_t)m * (S.maxN + 2);
-  unsigned* maxb = S.d_maxb + (size_t)m * (S.maxN + 2);
+  unsigned* maxf = S.d_maxf + (size_t)m * (S.maxN + 2) * kMaxSlots;
+  unsigned* maxb = S.d_maxb + (size_t)m * (S.maxN + 2) * kMaxSlots;
+  float* store = (float*)S.d_store;
+  float* dstore = (float*)W.dstore.p;
+  float* pp[2] = {(float*)S.d_pp, (float*)S.d_pp + Mn};
+  float* dpp[2] = {(float*)W.dpp.p, (float*)W.dpp.p + Mn};
+  float* dacc = (float*)W.dacc.p;
+  auto daccp = [&](int t) { return dacc + ((size_t)m * S.ns + t) * Mn; };
+  auto gp = [&](int t) { return g32 + (size_t)t * Mn; };
+  auto vp = [&](int t) { return vt + (size_t)t * Mn; };
+  auto slot = [&](int i) { return store + (size_t)i * Mn; };
+  auto dslot = [&](int i) { return dstore + (size_t)i * Mn; };
+  const int N = mo.N;
+  const ChainPlan& plan = mo.plan;
+  for (int t = 0; t < S.ns; t++)
+    if (mo.uses[t]) zero<float>(daccp(t), Mn, st);
+  zero<unsigned>(maxf, (S.maxN + 2) * kMaxSlots, st);
+  zero<unsigned>(maxb, (S.maxN + 2) * kMaxSlots, st);
+  auto fsc = [&](int step, int pair) { return RankScale{maxf, lnf, lnf, scal + MS_REF, step, pair}; };
+  auto bsc = [&](int step, int pair) { return RankScale{maxb, lnb, lnf, scal + MS_REF, step, pair}; };
+  auto state_of_rank = [&](int s) { return mo.seq[s - 1]; };
+  auto nosplit = [](int) { return false; };
+  auto launch = [&](int sa, int sb, int in_slot_or_pp, bool in_is_pp, int t, unsigned* maxv, double* lnv,
+                    float* const* outs, float* const* douts, const int* pslots, const int* pidx, bool connect) {
+    B3Args<float> a{};
+    a.in = in_is_pp ? pp[in_slot_or_pp] : slot(in_slot_or_pp);
+    a.din = in_is_pp ? dpp[in_slot_or_pp] : dslot(in_slot_or_pp);
+    a.g = gp(t);
+    a.v = vp(t);
+    a.nr = sb - sa + 1;
+    for (int k = 0; k < a.nr; k++) {
+      a.out[k] = outs[k];
+      a.dout[k] = douts[k];
+      if (connect) {
+        a.pair[k] = slot(pslots[k]);
+        a.dpair[k] = dslot(pslots[k]);
+        a.pidx[k] = pidx[k];
+      }
+    }
+    a.acc = connect ? daccp(t) : nullptr;
+    a.maxv = maxv;
+    a.lnv = lnv;
+    a.lnpair = lnf;
+    a.ref = scal + MS_REF;
+    a.gmax = S.d_gmax + t;
+    a.s0 = sa - 1;
+    S.prof.begin(connect ? 3 : 2, SFX_FP32, st, a.nr);
+    launch_block3<float>(L, a, true, true, connect, st);
+    S.prof.end(st);
+  };
+  k_tinit<float><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0));
+  {
+    int prev = 0, parity = 0;
+    chain_blocks(2, plan.n, 2, state_of_rank, nosplit, [&](int sa, int sb) {
+      float* outs[kB3Ranks] = {};
+      float* douts[kB3Ranks] = {};
+      int last = -1;
+      for (int s = sa; s <= sb; s++) {
+        int d = -1;
+        if (!plan.c)
+          d = s - 1;
+        else if ((s - 1) % plan.c == 0)
+          d = (s - 1) / plan.c;
+        else if (s == sb)
+          d = plan.seg(parity);
+        outs[s - sa] = d >= 0 ? slot(d) : nullptr;
+        douts[s - sa] = d >= 0 ? dslot(d) : nullptr;
+        if (s == sb) last = d;
+      }
+      launch(sa, sb, prev, false, mo.seq[sa - 1], maxf, lnf, outs, douts, nullptr, nullptr, false);
+      if (plan.c && last == plan.seg(parity)) parity ^= 1;
+      prev = last;
+    });
+  }
+  int resident = -1;
+  auto get = [&](int s) -> int {
+    if (!plan.c) return s - 1;
+    const int j = (s - 1) / plan.c, start = 1 + j * plan.c;
+    if (s == start) return j;
+    if (resident != j) {
+      const int end = std::min(start + plan.c - 1, plan.n);
+      int prev = j;
+      chain_blocks(start + 1, end, 2, state_of_rank, nosplit, [&](int sa, int sb) {
+        float* outs[kB3Ranks] = {};
+        float* douts[kB3Ranks] = {};
+        for (int s2 = sa; s2 <= sb; s2++) {
+          outs[s2 - sa] = slot(plan.seg(s2 - start - 1));
+          douts[s2 - sa] = dslot(plan.seg(s2 - start - 1));
+        }
+        launch(sa, sb, prev, false, mo.seq[sa - 1], maxf, lnf, outs, douts, nullptr, nullptr, false);
+        prev = plan.seg(sb - start - 1);
+      });
+      resident = j;
+    }
+    return plan.seg(s - start - 1);
+  };
+  auto segment = [&](int s) { return plan.c ? (s - 1) / plan.c : 0; };
+  int cur = 0;
+  const float* fin = nullptr;
+  const float* dfin = nullptr;
+  if (mo.kind == 1) {
+    k_set_ref2<<<1, 1, 0, st>>>(scal, lnf, N, 1.0);
+    const int tN = mo.seq[N - 1];
+    k_tinit<float><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(tN), vp(tN), pp[0], dpp[0], bsc(1, N));
+    const int iN = get(N);
+    k_tconnect<float, true><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[0], dpp[0], slot(iN), dslot(iN), daccp(tN),
+                                                             bsc(1, N), 1.0);
+    auto rank_of = [&](int k) { return N - k + 1; };
+    chain_blocks(
+        2, N, 2, [&](int k) { return mo.seq[rank_of(k) - 1]; },
+        [&](int k) { return segment(rank_of(k)) != segment(rank_of(k - 1)); },
+        [&](int ka, int kb) {
+          float* outs[kB3Ranks] = {};
+          float* douts[kB3Ranks] = {};
+          int pslots[kB3Ranks] = {}, pidx[kB3Ranks] = {};
+          for (int k = ka; k <= kb; k++) {
+            pslots[k - ka] = get(rank_of(k));
+            pidx[k - ka] = rank_of(k);
+          }
+          outs[kb - ka] = pp[cur ^ 1];
+          douts[kb - ka] = dpp[cur ^ 1];
+          launch(ka, kb, cur, true, mo.seq[rank_of(ka) - 1], maxb, lnb, outs, douts, pslots, pidx, true);
+          cur ^= 1;
+        });
+  } else {
+    const int h = N / 2;
+    k_set_ref2<<<1, 1, 0, st>>>(scal, lnf, h, 2.0);
+    const int ih = get(h);
+    launch_copy_interior<float>(L, slot(ih), pp[0], st);
+    launch_copy_interior<float>(L, dslot(ih), dpp[0], st);
+    if (N % 2 == 1) {
+      const int s = h + 1, t = mo.seq[s - 1];
+      tprop<float>(L, pp[0], dpp[0], gp(t), vp(t), pp[1], dpp[1], nullptr, nullptr, nullptr, fsc(s, 0), st);
+      cur = 1;
+      k_tconnect<float, true><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[cur], dpp[cur], pp[cur], dpp[cur], daccp(t),
+                                                               fsc(s, s), 0.5);
+    }
+    auto pair_of = [&](int s) { return N - s + 1; };
+    chain_blocks(
+        (N + 3) / 2, N, 2, state_of_rank, [&](int s) { return segment(pair_of(s)) != segment(pair_of(s - 1)); },
+        [&](int sa, int sb) {
+          float* outs[kB3Ranks] = {};
+          float* douts[kB3Ranks] = {};
+          int pslots[kB3Ranks] = {}, pidx[kB3Ranks] = {};
+          for (int s = sa; s <= sb; s++) {
+            pslots[s - sa] = get(pair_of(s));
+            pidx[s - sa] = pair_of(s);
+          }
+          outs[sb - sa] = pp[cur ^ 1];
+          douts[sb - sa] = dpp[cur ^ 1];
+          launch(sa, sb, cur, true, mo.seq[sa - 1], maxf, lnf, outs, douts, pslots, pidx, true);
+          cur ^= 1;
+        });
+  }
+  fin = pp[cur];
+  dfin = dpp[cur];
+  unsigned nb = grid1(L.V, kTB);
+  k_wsum2<float><<<nb, kTB, 0, st>>>(L, fin, dfin, S.d_partial, W.partial2.p);
+  k_tmol_finalize<<<1, 256, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, (int)nb, S.d_molscal, W.dmol.p, S.nm);
+}
+
+template <typename T>
+static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const T* g32or64, const double* v64) {
+  if constexpr (std::is_same<T, float>::value) {
+    const int kind = S.mols[m].kind;
+    if (scaled && block3_ok(S.lat.d) && (kind == 1 || kind == 2)) {
+      tsweep_blocked(S, W, m, vt, g32or64);
+      return;
+    }
+  }
+  const DevLattice& L = S.lat.d;
+  const long long Mn = L.M;
+  Molecule& mo = S.mols[m];
+  hipStream_t st = S.stream;
+  double* scal = S.d_molscal + (size_t)m * MS_COUNT;
+  double* lnf = S.d_lnf + (size_t)m * (S.maxN + 2);
+  double* lnb = S.d_lnb + (size_t)m * (S.maxN + 2);
+  unsigned* maxf = S.d_maxf + (size_t)m * (S.maxN + 2) * kMaxSlots;
+  unsigned* maxb = S.d_maxb + (size_t)m * (S.maxN + 2) * kMaxSlots;
   T* store = (T*)S.d_store;
   T* dstore = (T*)W.dstore.p;
   T* pp[2] = {(T*)S.d_pp, (T*)S.d_pp + Mn};
@@ -422,8 +618,8 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
     if (mo.uses[t]) zero<T>(daccp(t), Mn, st);
   RankScale none{nullptr, nullptr, nullptr, nullptr, 0, 0};
   if (scaled) {
-    zero<unsigned>(maxf, S.maxN + 2, st);
-    zero<unsigned>(maxb, S.maxN + 2, st);
+    zero<unsigned>(maxf, (S.maxN + 2) * kMaxSlots, st);
+    zero<unsigned>(maxb, (S.maxN + 2) * kMaxSlots, st);
   }
   auto fsc = [&](int step, int pair) { return scaled ? RankScale{maxf, lnf, lnf, scal + MS_REF, step, pair} : none; };
   auto bsc = [&](int step, int pair) { return scaled ? RankScale{maxb, lnb, lnf, scal + MS_REF, step, pair} : none; };
@@ -461,12 +657,12 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
       S.prof.end(st);
     }
     int nb = launch_wsum6<T>(L, pp6[(N - 1) & 1], dpp6[(N - 1) & 1], S.d_partial, W.partial2.p, st);
-    k_tmol_finalize<<<1, 1, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, nb, S.d_molscal, dm, S.nm);
+    k_tmol_finalize<<<1, 256, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, nb, S.d_molscal, dm, S.nm);
   } else if (mo.kind == 0) {
     int t = mo.seq[0];
     unsigned nb = grid1(L.V, kTB);
     k_wsum_mono<<<nb, kTB, 0, st>>>(L, S.d_swf + (size_t)t * Mn, v64 + (size_t)t * Mn, S.d_partial, W.partial2.p);
-    k_tmol_finalize<<<1, 1, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, (int)nb, S.d_molscal, dm, S.nm);
+    k_tmol_finalize<<<1, 256, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, (int)nb, S.d_molscal, dm, S.nm);
   } else {
     auto slot = [&](int i) { return store + (size_t)i * Mn; };
     auto dslot = [&](int i) { return dstore + (size_t)i * Mn; };
@@ -532,7 +728,7 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
     }
     unsigned nb = grid1(L.V, kTB);
     k_wsum2<T><<<nb, kTB, 0, st>>>(L, fin, dfin, S.d_partial, W.partial2.p);
-    k_tmol_finalize<<<1, 1, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, (int)nb, S.d_molscal, dm, S.nm);
+    k_tmol_finalize<<<1, 256, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, (int)nb, S.d_molscal, dm, S.nm);
   }
 }
 
@@ -567,6 +763,7 @@ void jvp_at_base(System& S, const double* v, double* jv, int precision) {
   bool fp32 = precision == SFX_FP32;
   float* v32 = (float*)W.v32.p;
   if (fp32) k_state_dir<float><<<grid1(Mn, kTB), kTB, 0, st>>>(Mn, S.ns, S.d_st_itvar, v, v32);
+  if (fp32 && block3_ok(L)) launch_gmax(L, S.d_swf32, S.ns, S.d_gmax, st);
   for (int pass = 0; pass < 2; pass++)
     for (int m = 0; m < S.nm; m++) {
       bool solv = S.mols[m].freedom == SFX_SOLVENT;

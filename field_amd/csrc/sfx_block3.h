@@ -1,0 +1,63 @@
+// sfx_block3.h -- several propagator ranks per launch on the 3-D simple-cubic
+// lattice (temporal blocking through LDS tiles).
+#pragma once
+
+#include "sfx_kernels.h"
+
+namespace sfx {
+
+constexpr int kB3Ranks = 4;  // ranks per launch at most
+
+// Ranks s0+1 .. s0+nr of one chain, all of one segment state.
+template <typename T>
+struct B3Args {
+  const T* in;                 // rank s0 (stored scale)
+  const T* din;                // its tangent (tangent sweeps)
+  const T* g;                  // SWF of the state
+  const T* v;                  // tangent direction of the state
+  T* out[kB3Ranks];            // where rank s0+k goes (nullptr: not kept)
+  T* dout[kB3Ranks];
+  const T* pair[kB3Ranks];     // composition partner of rank s0+k
+  const T* dpair[kB3Ranks];
+  T* acc;                      // accumulator of the state (phi, or dphi for tangents)
+  int nr;
+  // FP32 log-scaling: the input is divided by max(rank s0) (maxv[s0]) and
+  // every rank by the lattice maximum of g (gmax); lnv[s] is the log of the
+  // divisor accumulated up to rank s, lnpair[pidx[k]] that of the partner
+  unsigned* maxv;
+  double* lnv;
+  const double* lnpair;
+  const double* ref;
+  const float* gmax;
+  int s0;
+  int pidx[kB3Ranks];
+};
+
+// Splits chain steps a..b (ascending) into launches: a launch ends at a grid
+// line ((step - g0 + 1) % kB3Ranks == 0), before a step of another segment
+// state, or before a step that split() marks. f(first, last) per launch. The
+// grid makes the partition of any stretch that starts on a grid line equal
+// to that of the whole chain, so a recomputed stretch reproduces it bit for bit.
+template <class State, class Split, class F>
+void chain_blocks(int a, int b, int g0, State state, Split split, F f) {
+  int s = a;
+  while (s <= b) {
+    int e = s;
+    while (e < b && (e + 1 - g0) % kB3Ranks != 0 && state(e + 1) == state(s) && !split(e + 1)) e++;
+    f(s, e);
+    s = e + 1;
+  }
+}
+
+// true when the blocked sweep applies: 3-D simple cubic, first order, every
+// side at least kB3Ranks + 1 layers (one reflection reaches the interior)
+bool block3_ok(const DevLattice& L);
+
+template <typename T>
+void launch_block3(const DevLattice& L, const B3Args<T>& a, bool tangent, bool scaled, bool connect,
+                   hipStream_t st);
+
+// max over the interior of each state's FP32 SWF, into gmax[ns] (zeroed here)
+void launch_gmax(const DevLattice& L, const float* swf32, int ns, float* gmax, hipStream_t st);
+
+}  // namespace sfx

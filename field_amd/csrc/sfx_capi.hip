@@ -181,6 +181,7 @@ int sfx_solver_defaults(sfx_solver_opts* o) {
   o->jvp_precision = SFX_FP32;
   o->fp64_switch = 1e-3;
   o->precondition = 1;
+  o->ptc_dt0 = 1.0;
   o->verbose = 0;
   return SFX_OK;
 }
@@ -194,7 +195,7 @@ int sfx_precondition(sfx_system* s, const double* r, double* out, int where) {
     if (!S.precond) S.precond = precond_create(S);
     DevBuf<double> a((size_t)S.nvar), b((size_t)S.nvar);
     load_vec(S, r, where, a.p);
-    precond_apply(S.precond, S, a.p, b.p);
+    precond_apply(S.precond, S, a.p, b.p, 0.0);
     S.store_f(b.p, out, where);
   });
 }
@@ -242,15 +243,19 @@ int sfx_profile_stop(sfx_system* s, sfx_profile_report* rep) {
         sfx_profile_entry& e = rep->entries[k * 2 + pr];
         e.kind = k;
         e.precision = pr;
-        e.bytes_per_launch = words[k] * (pr == SFX_FP32 ? 4.0 : 8.0) * (double)S.lat.d.V;
       }
     for (const Profiler::Mark& m : p.marks) {
       float ms = 0;
       SFX_HIP(hipEventElapsedTime(&ms, p.pool[m.ev], p.pool[m.ev + 1]));
       sfx_profile_entry& e = rep->entries[m.kind * 2 + m.prec];
       e.launches++;
+      e.ranks += m.ranks;
       e.total_ms += ms;
     }
+    for (sfx_profile_entry& e : rep->entries)
+      if (e.launches)
+        e.bytes_per_launch = words[e.kind] * (e.precision == SFX_FP32 ? 4.0 : 8.0) * (double)S.lat.d.V *
+                             (double)e.ranks / (double)e.launches;
     rep->count = 2 * SFX_K_COUNT;
     p.marks.clear();
     p.used = 0;

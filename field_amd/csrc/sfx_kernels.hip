@@ -86,7 +86,7 @@ __global__ void k_init_rank(DevLattice L, const T* __restrict__ g, T* __restrict
   if (sc.maxv) {
     m = block_max(m);
     if (threadIdx.x == 0) {
-      atomicMax(sc.maxv + sc.step, __float_as_uint(m));
+      put_step_max(sc.maxv, sc.step, m);
       if (blockIdx.x == 0) sc.lnv[sc.step] = 0.0;
     }
   }
@@ -94,7 +94,7 @@ __global__ void k_init_rank(DevLattice L, const T* __restrict__ g, T* __restrict
 
 // scale factor of step `step`: out values are multiplied by 1/max(previous step)
 __device__ __forceinline__ float step_scale(const RankScale& sc, double& lncum) {
-  float mprev = __uint_as_float(sc.maxv[sc.step - 1]);
+  float mprev = get_step_max(sc.maxv, sc.step - 1);
   if (!(mprev > 0.f)) mprev = 1.f;  // all-zero propagator: keep it unscaled
   lncum = sc.lnv[sc.step - 1] + log((double)mprev);
   return 1.f / mprev;
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(kBlock) k_prop(DevLattice L, const T* __restri
   if constexpr (SCALED) {
     m = block_max(m);
     if (threadIdx.x == 0) {
-      atomicMax(sc.maxv + sc.step, __float_as_uint(m));
+      put_step_max(sc.maxv, sc.step, m);
       if (blockIdx.x == 0) sc.lnv[sc.step] = lncum;
     }
   }
@@ -178,9 +178,16 @@ __global__ void k_wsum(DevLattice L, const T* __restrict__ G, double* __restrict
 __global__ void k_mol_finalize(MolParams mp, const double* __restrict__ partial, int nparts,
                                double* __restrict__ scal, const double* __restrict__ lnfN,
                                const double* __restrict__ allscal, int nmol) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  // the partial sums, reduced by the whole block in a fixed order
+  __shared__ double red[256 / 64];
   double s = 0;
-  for (int i = 0; i < nparts; i++) s += partial[i];
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  s = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i];
   double lnsum = mp.kind == 3 ? log(s / 6.0) : log(s);  // second order: Lat3D2ndO.cpp:1024
   double lnscale = lnfN ? *lnfN : 0.0;  // FP32: log-scale of the final rank
   double lnGN = lnsum + lnscale;
@@ -464,7 +471,7 @@ int launch_wsum(const DevLattice& L, const T* G, double* partial, hipStream_t st
 
 void launch_mol_finalize(const MolParams& mp, const double* partial, int nparts, double* scal,
                          const double* lnfN, const double* allscal, int nmol, hipStream_t st) {
-  k_mol_finalize<<<1, 64, 0, st>>>(mp, partial, nparts, scal, lnfN, allscal, nmol);
+  k_mol_finalize<<<1, 256, 0, st>>>(mp, partial, nparts, scal, lnfN, allscal, nmol);
 }
 
 template <typename A>

@@ -8,7 +8,13 @@
 // step (iterative refinement). Backtracking on ||f|| globalises the step.
 // Where it applies, GMRES is right-preconditioned with the Fourier-space
 // Jacobian of the homogeneous melt (sfx_precond.hip): it solves (J P) z = -f
-// and steps dx = P z.
+// and steps dx = P z. With ptc_dt0 > 0 the step is pseudo-transient
+// continuation instead: (I/dt + J) dx = -f with the preconditioner shifted
+// alike, dt grown by the ratio of successive ||f|| (switched evolution
+// relaxation), a step that raises ||f|| fivefold or gives a non-finite
+// residual rejected and retried with dt/4. Small dt follows the relaxation
+// dx/dt = -f that fixed-point iterations such as DIIS follow; large dt is
+// Newton.
 #include <cmath>
 
 #include "sfx_solve.h"
@@ -29,14 +35,18 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     if (!S.precond) S.precond = precond_create(S);
     pc = S.precond;
   }
-  // the GMRES operator v -> J P v (J v without a preconditioner)
+  const bool ptc = o.ptc_dt0 > 0;
+  double dt = ptc ? o.ptc_dt0 : 0.0;
+  double sigma = ptc ? 1.0 / dt : 0.0;
+  // the GMRES operator v -> (sigma I + J) P v (without a preconditioner P = I)
   auto op = [&](const double* v, double* out) {
+    const double* a = v;
     if (pc) {
-      precond_apply(pc, S, v, pz.p);
-      jvp_at_base(S, pz.p, out, o.jvp_precision);
-    } else {
-      jvp_at_base(S, v, out, o.jvp_precision);
+      precond_apply(pc, S, v, pz.p, sigma);
+      a = pz.p;
     }
+    jvp_at_base(S, a, out, o.jvp_precision);
+    if (sigma != 0.0) V.axpby(sigma, a, 1.0, out);
     st.jvp_evals++;
   };
   auto Vk = [&](int k) { return basis.p + (size_t)k * n; };
@@ -169,7 +179,25 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     }
     if (pc) {  // dx = P z
       V.copy(dx.p, w.p);
-      precond_apply(pc, S, w.p, dx.p);
+      precond_apply(pc, S, w.p, dx.p, sigma);
+    }
+    if (ptc) {
+      V.copy(x, x0.p);
+      V.axpby(1.0, dx.p, 1.0, x);
+      const double rt = eval(prec);
+      if (!std::isfinite(rt) || rt > 5.0 * rn) {
+        V.copy(x0.p, x);
+        rn = eval(prec);  // back to the last point (and its base state for the JVPs)
+        dt *= 0.25;
+        if (o.verbose) std::fprintf(stderr, "  ptc step rejected (|f|=%.3e), dt=%.3e\n", rt, dt);
+        if (dt < 1e-8) break;
+      } else {
+        dt = std::min(dt * rn / rt, 1e12);
+        rn = rt;
+        if (o.verbose) std::fprintf(stderr, "  ptc dt=%.3e\n", dt);
+      }
+      sigma = 1.0 / dt;
+      continue;
     }
     // ---- backtracking line search on ||f|| ----
     V.copy(x, x0.p);

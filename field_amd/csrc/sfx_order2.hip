@@ -24,7 +24,8 @@ __device__ __forceinline__ float wmax2(float v) {
 }
 
 // neighbour (x,y,z) + d along axis a (d = +-1): returns the site index and
-// whether the halo read reverses the axis' components. An axis beyond the
+// whether the halo read reverses the axis' components, or -1 across a surface
+// bound (the frozen surface layer carries no propagator). An axis beyond the
 // lattice's gradients is homogeneous: the neighbour is the site itself (the
 // 1-D and 2-D lattices lump those directions, Lat1D2ndO.cpp:32-62,
 // Lat2D2ndO.cpp:31-92).
@@ -35,9 +36,11 @@ __device__ __forceinline__ long long nb2(const DevLattice& L, int x, int y, int 
   if (a >= L.grad) return sidx(L, x, y, z);
   int v = c[a] + d;
   if (v == 0) {
+    if (L.surf1[a]) return -1;
     v = L.b1[a];
     flip = L.flip1[a];
   } else if (v == n[a] + 1) {
+    if (L.surf2[a]) return -1;
     v = L.b2[a];
     flip = L.flip2[a];
   }
@@ -66,7 +69,7 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
   T scale = T(1);
   double lncum = 0;
   if constexpr (SCALED) {
-    float mp = __uint_as_float(sc.maxv[sc.step - 1]);
+    float mp = get_step_max(sc.maxv, sc.step - 1);
     if (!(mp > 0.f)) mp = 1.f;
     lncum = sc.lnv[sc.step - 1] + log((double)mp);
     scale = (T)(1.f / mp);
@@ -85,7 +88,7 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
       long long nbi = nb2(L, x, y, z, kAxis[k], -kSign[k], flip);
       T in[6], yk[6];
 #pragma unroll
-      for (int j = 0; j < 6; j++) in[j] = Gin[(long long)j * M + nbi];
+      for (int j = 0; j < 6; j++) in[j] = nbi < 0 ? T(0) : Gin[(long long)j * M + nbi];
       transfer_rows(S, Lw, in, yk);
       T a = flip ? yk[5 - k] : yk[k];
       T o = gv * a;
@@ -94,7 +97,7 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
       if constexpr (TANGENT) {
         T din[6], dyk[6];
 #pragma unroll
-        for (int j = 0; j < 6; j++) din[j] = dGin[(long long)j * M + nbi];
+        for (int j = 0; j < 6; j++) din[j] = nbi < 0 ? T(0) : dGin[(long long)j * M + nbi];
         transfer_rows(S, Lw, din, dyk);
         T da = flip ? dyk[5 - k] : dyk[k];
         dGout[(long long)k * M + r] = gv * (da - vv * a);
@@ -108,7 +111,7 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmaxf(m, sm[i]);
-      atomicMax(sc.maxv + sc.step, __float_as_uint(m));
+      put_step_max(sc.maxv, sc.step, m);
       if (blockIdx.x == 0) sc.lnv[sc.step] = lncum;
     }
   }
@@ -127,7 +130,7 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
   T scale = T(1), cfac = T(1);
   double lncum = 0;
   if constexpr (SCALED) {
-    float mp = __uint_as_float(sc.maxv[sc.step - 1]);
+    float mp = get_step_max(sc.maxv, sc.step - 1);
     if (!(mp > 0.f)) mp = 1.f;
     lncum = sc.lnv[sc.step - 1] + log((double)mp);
     scale = (T)(1.f / mp);
@@ -145,8 +148,8 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
       bool flip;
       long long nbi = nb2(L, x, y, z, kAxis[j], kSign[j], flip);
       int jj = flip ? 5 - j : j;
-      zz[j] = Gin[(long long)jj * M + nbi];
-      if constexpr (TANGENT) dz[j] = dGin[(long long)jj * M + nbi];
+      zz[j] = nbi < 0 ? T(0) : Gin[(long long)jj * M + nbi];
+      if constexpr (TANGENT) dz[j] = nbi < 0 ? T(0) : dGin[(long long)jj * M + nbi];
     }
     T o[6], d[6];
     transfer_rows(S, Lw, zz, o);
@@ -179,7 +182,7 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmaxf(m, sm[i]);
-      atomicMax(sc.maxv + sc.step, __float_as_uint(m));
+      put_step_max(sc.maxv, sc.step, m);
       if (blockIdx.x == 0) sc.lnv[sc.step] = lncum;
     }
   }
@@ -225,7 +228,7 @@ __global__ void k_init2(DevLattice L, const T* __restrict__ g, const T* __restri
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmaxf(m, sm[i]);
-      atomicMax(sc.maxv + sc.step, __float_as_uint(m));
+      put_step_max(sc.maxv, sc.step, m);
       if (blockIdx.x == 0) sc.lnv[sc.step] = 0.0;
     }
   }

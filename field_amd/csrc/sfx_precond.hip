@@ -12,9 +12,11 @@
 //   du'_i      = du_i - sum_j chi_ij (a dphi_j - phi0_j dphi_tot)
 //   df_i       = -dphi_tot + du'_i - mean_k du'_k          (SF_Solve.cpp:554-590)
 // and at q = 0 the restricted normalisation keeps dphi = 0, so J0(0) is the
-// projection I - 11^T/n. The preconditioner applies J0(q)^-1 (the
-// pseudo-inverse at q = 0): pack the residual's interior into complex
-// fields, DFT along z, y, x, multiply, inverse DFT, unpack. The DFTs are
+// projection I - 11^T/n. The preconditioner applies (sigma I + J0(q))^-1
+// (sigma = 1/dt of pseudo-transient continuation, 0 for plain Newton; the
+// pseudo-inverse at q = 0 when sigma = 0): pack the residual's interior into
+// complex fields, DFT along z, y, x, solve the small block per q, inverse
+// DFT, unpack. The DFTs are
 // direct sums over a line with a twiddle table (any lattice size); one
 // thread per output element, consecutive threads along the innermost
 // contiguous axis so every load is coalesced.
@@ -71,21 +73,60 @@ __global__ void k_pack(DevLattice L, const double* __restrict__ r, int nv, cf* _
   out[q] = cf{(float)r[(long long)f * L.M + sidx(L, x, y, z)], 0.f};
 }
 
-// y(q) = Jinv(q) r(q), Jinv stored [i][j][V]
-__global__ void k_apply_block(const cf* __restrict__ in, cf* __restrict__ out, const float* __restrict__ jinv,
-                              int nv, long long V) {
+// y(q) = (sigma I + J0(q))^-1 r(q), J0 stored [i][j][V]; Gauss-Jordan on the
+// real block with partial pivoting, pivots kept at least 1e-6 in magnitude
+__global__ void k_apply_block(const cf* __restrict__ in, cf* __restrict__ out, const float* __restrict__ j0,
+                              int nv, long long V, float sigma) {
   long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= V) return;
+  float A[SFX_MAXS][SFX_MAXS], re[SFX_MAXS], im[SFX_MAXS];
   for (int i = 0; i < nv; i++) {
-    float re = 0.f, im = 0.f;
-    for (int j = 0; j < nv; j++) {
-      const float a = jinv[((long long)i * nv + j) * V + q];
-      const cf v = in[(long long)j * V + q];
-      re += a * v.x;
-      im += a * v.y;
-    }
-    out[(long long)i * V + q] = cf{re, im};
+    for (int j = 0; j < nv; j++) A[i][j] = j0[((long long)i * nv + j) * V + q] + (i == j ? sigma : 0.f);
+    const cf v = in[(long long)i * V + q];
+    re[i] = v.x;
+    im[i] = v.y;
   }
+  if (q == 0 && sigma == 0.f) {  // the projection is its own pseudo-inverse
+    float mr = 0.f, mi = 0.f;
+    for (int i = 0; i < nv; i++) {
+      mr += re[i] / nv;
+      mi += im[i] / nv;
+    }
+    for (int i = 0; i < nv; i++) out[(long long)i * V + q] = cf{re[i] - mr, im[i] - mi};
+    return;
+  }
+  for (int c = 0; c < nv; c++) {
+    int p = c;
+    for (int r = c + 1; r < nv; r++)
+      if (fabsf(A[r][c]) > fabsf(A[p][c])) p = r;
+    if (p != c) {
+      for (int j = 0; j < nv; j++) {
+        const float tmp = A[c][j];
+        A[c][j] = A[p][j];
+        A[p][j] = tmp;
+      }
+      float tmp = re[c];
+      re[c] = re[p];
+      re[p] = tmp;
+      tmp = im[c];
+      im[c] = im[p];
+      im[p] = tmp;
+    }
+    float d = A[c][c];
+    if (fabsf(d) < 1e-6f) d = d < 0.f ? -1e-6f : 1e-6f;
+    const float inv = 1.f / d;
+    for (int j = 0; j < nv; j++) A[c][j] *= inv;
+    re[c] *= inv;
+    im[c] *= inv;
+    for (int r = 0; r < nv; r++) {
+      if (r == c) continue;
+      const float f = A[r][c];
+      for (int j = 0; j < nv; j++) A[r][j] -= f * A[c][j];
+      re[r] -= f * re[c];
+      im[r] -= f * im[c];
+    }
+  }
+  for (int i = 0; i < nv; i++) out[(long long)i * V + q] = cf{re[i], im[i]};
 }
 
 // real part / V -> [itvar][M] interior (halos and off-site entries 0)
@@ -122,7 +163,7 @@ struct Precond {
   long long V = 0;
   int n[3] = {0, 0, 0};
   DevBuf<cf> b0, b1, tw;      // tw: forward then inverse table per axis
-  DevBuf<float> jinv;
+  DevBuf<float> j0;
   long long twoff[3][2] = {};
 
   void init(const System& S) {
@@ -164,14 +205,14 @@ struct Precond {
     for (int i = 0; i < ns; i++)
       for (int j = 0; j < ns; j++) chi[i][j] = S.chi[(size_t)i * ns + j];
     std::vector<float> hj((size_t)nv * nv * V);
-    std::vector<double> Sq((size_t)ns * ns), J((size_t)nv * nv), inv((size_t)nv * nv), dphi(ns), dup(nv);
+    std::vector<double> Sq((size_t)ns * ns), J((size_t)nv * nv), dphi(ns), dup(nv);
     for (int qx = 0; qx < n[0]; qx++)
       for (int qy = 0; qy < n[1]; qy++)
         for (int qz = 0; qz < n[2]; qz++) {
           const long long q = ((long long)qx * n[1] + qy) * n[2] + qz;
           if (q == 0) {
             for (int i = 0; i < nv; i++)
-              for (int j = 0; j < nv; j++) inv[(size_t)i * nv + j] = (i == j ? 1.0 : 0.0) - 1.0 / nv;
+              for (int j = 0; j < nv; j++) J[(size_t)i * nv + j] = (i == j ? 1.0 : 0.0) - 1.0 / nv;
           } else {
             const double a = (std::cos(2 * M_PI * qx / n[0]) + std::cos(2 * M_PI * qy / n[1]) +
                               std::cos(2 * M_PI * qz / n[2])) / 3.0;
@@ -200,45 +241,12 @@ struct Precond {
               }
               for (int i = 0; i < nv; i++) J[(size_t)i * nv + col] = -tot + dup[i] - mean;
             }
-            invert(J, inv, nv);
           }
           for (int i = 0; i < nv; i++)
-            for (int j = 0; j < nv; j++) hj[((size_t)i * nv + j) * V + q] = (float)inv[(size_t)i * nv + j];
+            for (int j = 0; j < nv; j++) hj[((size_t)i * nv + j) * V + q] = (float)J[(size_t)i * nv + j];
         }
-    jinv.alloc(hj.size());
-    SFX_HIP(hipMemcpy(jinv.p, hj.data(), hj.size() * sizeof(float), hipMemcpyHostToDevice));
-  }
-
-  // Gauss-Jordan with partial pivoting; a singular block gets its singular
-  // directions left alone (zero rows of the pivot stay zero)
-  static void invert(const std::vector<double>& Ain, std::vector<double>& X, int k) {
-    std::vector<double> A(Ain);
-    X.assign((size_t)k * k, 0.0);
-    for (int i = 0; i < k; i++) X[(size_t)i * k + i] = 1.0;
-    for (int c = 0; c < k; c++) {
-      int piv = c;
-      for (int r = c + 1; r < k; r++)
-        if (std::fabs(A[(size_t)r * k + c]) > std::fabs(A[(size_t)piv * k + c])) piv = r;
-      for (int j = 0; j < k; j++) {
-        std::swap(A[(size_t)c * k + j], A[(size_t)piv * k + j]);
-        std::swap(X[(size_t)c * k + j], X[(size_t)piv * k + j]);
-      }
-      double d = A[(size_t)c * k + c];
-      if (std::fabs(d) < 1e-12) d = d < 0 ? -1e-12 : 1e-12;
-      for (int j = 0; j < k; j++) {
-        A[(size_t)c * k + j] /= d;
-        X[(size_t)c * k + j] /= d;
-      }
-      for (int r = 0; r < k; r++) {
-        if (r == c) continue;
-        const double f = A[(size_t)r * k + c];
-        if (f == 0) continue;
-        for (int j = 0; j < k; j++) {
-          A[(size_t)r * k + j] -= f * A[(size_t)c * k + j];
-          X[(size_t)r * k + j] -= f * X[(size_t)c * k + j];
-        }
-      }
-    }
+    j0.alloc(hj.size());
+    SFX_HIP(hipMemcpy(j0.p, hj.data(), hj.size() * sizeof(float), hipMemcpyHostToDevice));
   }
 
   void dft3(const System& S, cf* a, cf* b, int dir, cf*& result) {
@@ -251,12 +259,12 @@ struct Precond {
     result = b;
   }
 
-  void apply(const System& S, const double* r, double* out) {
+  void apply(const System& S, const double* r, double* out, double sigma) {
     const DevLattice& L = S.lat.d;
     k_pack<<<grid1(V * nv, kPB), kPB, 0, S.stream>>>(L, r, nv, b0.p);
     cf* res = nullptr;
     dft3(S, b0.p, b1.p, 0, res);  // result in b1
-    k_apply_block<<<grid1(V, kPB), kPB, 0, S.stream>>>(b1.p, b0.p, jinv.p, nv, V);
+    k_apply_block<<<grid1(V, kPB), kPB, 0, S.stream>>>(b1.p, b0.p, j0.p, nv, V, (float)sigma);
     dft3(S, b0.p, b1.p, 1, res);
     k_unpack<<<grid1(L.M * nv, kPB), kPB, 0, S.stream>>>(L, b1.p, nv, S.d_itmask, out);
     SFX_HIP(hipGetLastError());
@@ -273,7 +281,9 @@ Precond* precond_create(const System& S) {
   }
   return p;
 }
-void precond_apply(Precond* p, const System& S, const double* r, double* out) { p->apply(S, r, out); }
+void precond_apply(Precond* p, const System& S, const double* r, double* out, double sigma) {
+  p->apply(S, r, out, sigma);
+}
 void precond_free(Precond* p) { delete p; }
 
 }  // namespace sfx

@@ -16,13 +16,20 @@ namespace sfx {
 // pass.
 struct ChainPlan {
   int n = 0, c = 0, nck = 0;
-  void init(int n_, bool checkpoint) {
+  // gran: c is a multiple of it, so segments start where launches of several
+  // ranks start
+  void init(int n_, bool checkpoint, int gran = 1) {
     n = n_;
     c = 0;
     nck = 0;
     if (!checkpoint || n < 4) return;
     c = 1;
     while (c * c < n) c++;
+    c = (c + gran - 1) / gran * gran;
+    if (c >= n) {
+      c = 0;
+      return;
+    }
     nck = (n + c - 1) / c;
   }
   int slots() const { return c ? nck + std::max(2, c - 1) : n; }
@@ -90,9 +97,10 @@ struct Profiler {
   struct Mark {
     int kind, prec;
     size_t ev;
+    int ranks;
   };
   std::vector<Mark> marks;
-  void begin(int kind, int prec, hipStream_t st);
+  void begin(int kind, int prec, hipStream_t st, int ranks = 1);
   void end(hipStream_t st);
   ~Profiler();
 };
@@ -142,6 +150,7 @@ struct System {
   unsigned* d_maxf = nullptr;
   unsigned* d_maxb = nullptr;
   double* d_partial = nullptr;  // reduction partials
+  float* d_gmax = nullptr;      // ns: lattice maximum of each FP32 SWF (blocked sweeps)
   double* d_phib_state = nullptr;
   double* d_cbulk = nullptr;
   double* d_scalar = nullptr;   // misc device scalars
@@ -168,12 +177,14 @@ struct System {
 
   template <typename T, typename A>
   void sweep(int m, bool scaled);
+  template <typename T>
+  void sweep_blocked(int m, bool scaled);
 };
 
 // Fourier-space Newton-Krylov preconditioner (sfx_precond.hip)
 bool precond_supported(const System& S);
 Precond* precond_create(const System& S);
-void precond_apply(Precond* p, const System& S, const double* r, double* out);
+void precond_apply(Precond* p, const System& S, const double* r, double* out, double sigma);
 void precond_free(Precond* p);
 
 }  // namespace sfx

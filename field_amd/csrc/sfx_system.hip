@@ -7,7 +7,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
+#include "sfx_block3.h"
 #include "sfx_order2.h"
 #include "sfx_system.h"
 
@@ -62,7 +64,7 @@ static T* dalloc(size_t n) {
 System::~System() {
   void* ptrs[] = {d_u, d_f, d_swf, d_swf32, d_phi_m, d_acc, d_phi, d_rho, d_phitot, d_fprof, d_frozen,
                   d_st_itvar, d_mol_uses, d_itmask, d_sites, d_store, d_pp, d_molscal, d_lnf, d_lnb,
-                  d_maxf, d_maxb, d_partial, d_phib_state, d_cbulk, d_scalar, d_xabi};
+                  d_maxf, d_maxb, d_partial, d_phib_state, d_cbulk, d_scalar, d_xabi, d_gmax};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   tangent_free(tan);
@@ -230,7 +232,7 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
   checkpointed = budget != 0 && (size_t)maxstore * slot_bytes > budget;
   store_slots = 1;
   for (Molecule& mo : mols) {
-    mo.plan.init(mo.nstore, checkpointed);
+    mo.plan.init(mo.nstore, checkpointed, block3_ok(lat.d) ? kB3Ranks : 1);
     store_slots = std::max(store_slots, mo.plan.slots());
   }
   if (checkpointed)
@@ -241,10 +243,11 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
   d_molscal = dalloc<double>((size_t)nm * MS_COUNT);
   d_lnf = dalloc<double>((size_t)nm * (maxN + 2));
   d_lnb = dalloc<double>((size_t)nm * (maxN + 2));
-  d_maxf = dalloc<unsigned>((size_t)nm * (maxN + 2));
-  d_maxb = dalloc<unsigned>((size_t)nm * (maxN + 2));
+  d_maxf = dalloc<unsigned>((size_t)nm * (maxN + 2) * kMaxSlots);
+  d_maxb = dalloc<unsigned>((size_t)nm * (maxN + 2) * kMaxSlots);
   npart = (int)grid1(std::max<long long>(lat.d.M, lat.d.V), 256);
   d_partial = dalloc<double>(npart);
+  d_gmax = dalloc<float>(SFX_MAXS);
   d_phib_state = dalloc<double>(SFX_MAXS);
   d_cbulk = dalloc<double>(SFX_MAXS);
   d_scalar = dalloc<double>(16);
@@ -273,14 +276,14 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
 
 void System::sync() { SFX_HIP(hipStreamSynchronize(stream)); }
 
-void Profiler::begin(int kind, int prec, hipStream_t st) {
+void Profiler::begin(int kind, int prec, hipStream_t st, int ranks) {
   if (!on) return;
   while (pool.size() < used + 2) {
     hipEvent_t e;
     SFX_HIP(hipEventCreate(&e));
     pool.push_back(e);
   }
-  marks.push_back(Mark{kind, prec, used});
+  marks.push_back(Mark{kind, prec, used, ranks});
   SFX_HIP(hipEventRecord(pool[used], st));
   used += 2;
 }
@@ -336,8 +339,8 @@ void System::sweep(int m, bool scaled) {
   double* scal = d_molscal + (size_t)m * MS_COUNT;
   double* lnf = d_lnf + (size_t)m * (maxN + 2);
   double* lnb = d_lnb + (size_t)m * (maxN + 2);
-  unsigned* maxf = d_maxf + (size_t)m * (maxN + 2);
-  unsigned* maxb = d_maxb + (size_t)m * (maxN + 2);
+  unsigned* maxf = d_maxf + (size_t)m * (maxN + 2) * kMaxSlots;
+  unsigned* maxb = d_maxb + (size_t)m * (maxN + 2) * kMaxSlots;
   T* store = (T*)d_store;
   T* pp[2] = {(T*)d_pp, (T*)d_pp + Mn};
   A* acc = (A*)d_acc;
@@ -351,8 +354,8 @@ void System::sweep(int m, bool scaled) {
   for (int t = 0; t < ns; t++)
     if (mo.uses[t]) zero_bytes(accp(t), (size_t)Mn * sizeof(A), st);
   if (scaled) {
-    k_fill_u32<<<grid1(N + 2, 64), 64, 0, st>>>(maxf, 0u, N + 2);
-    k_fill_u32<<<grid1(N + 2, 64), 64, 0, st>>>(maxb, 0u, N + 2);
+    k_fill_u32<<<grid1((N + 2) * kMaxSlots, 64), 64, 0, st>>>(maxf, 0u, (N + 2) * kMaxSlots);
+    k_fill_u32<<<grid1((N + 2) * kMaxSlots, 64), 64, 0, st>>>(maxb, 0u, (N + 2) * kMaxSlots);
   }
   RankScale none{nullptr, nullptr, nullptr, nullptr, 0, 0};
   auto fsc = [&](int step, int pair) {
@@ -403,6 +406,12 @@ void System::sweep(int m, bool scaled) {
     launch_mol_finalize(mo.params(m), d_partial, nb, scal, nullptr, d_molscal, nm, st);
     launch_mol_phi<double>(L, nullptr, d_swf + (size_t)t * Mn, scal, 1, d_phi_m + ((size_t)m * ns + t) * Mn, st);
     return;
+  }
+  if constexpr (std::is_same<T, A>::value) {
+    if (block3_ok(L)) {
+      sweep_blocked<T>(m, scaled);
+      return;
+    }
   }
   auto slot = [&](int i) { return store + (size_t)i * Mn; };
   launch_init_rank<T>(L, gp(mo.seq[0]), slot(0), fsc(1, 0), st);
@@ -458,9 +467,175 @@ void System::sweep(int m, bool scaled) {
     if (mo.uses[t]) launch_mol_phi<A>(L, accp(t), d_swf + (size_t)t * Mn, scal, 0, d_phi_m + ((size_t)m * ns + t) * Mn, st);
 }
 
+// The sweep of sweep() for a first-order chain (full or half store) on the
+// 3-D simple-cubic lattice, several ranks per launch (sfx_block3.hip). The
+// zeroing of the accumulators and of the scale maxima has been done by sweep().
+template <typename T>
+void System::sweep_blocked(int m, bool scaled) {
+  const DevLattice& L = lat.d;
+  const long long Mn = L.M;
+  Molecule& mo = mols[m];
+  hipStream_t st = stream;
+  double* scal = d_molscal + (size_t)m * MS_COUNT;
+  double* lnf = d_lnf + (size_t)m * (maxN + 2);
+  double* lnb = d_lnb + (size_t)m * (maxN + 2);
+  unsigned* maxf = d_maxf + (size_t)m * (maxN + 2) * kMaxSlots;
+  unsigned* maxb = d_maxb + (size_t)m * (maxN + 2) * kMaxSlots;
+  T* store = (T*)d_store;
+  T* pp[2] = {(T*)d_pp, (T*)d_pp + Mn};
+  T* acc = (T*)d_acc;
+  auto accp = [&](int t) { return acc + ((size_t)m * ns + t) * Mn; };
+  auto gp = [&](int t) -> const T* {
+    if constexpr (sizeof(T) == 4) return (const T*)(d_swf32 + (size_t)t * Mn);
+    else return (const T*)(d_swf + (size_t)t * Mn);
+  };
+  auto slot = [&](int i) { return store + (size_t)i * Mn; };
+  const int N = mo.N;
+  const int P = sizeof(T) == 4 ? SFX_FP32 : SFX_FP64;
+  const ChainPlan& plan = mo.plan;
+  RankScale none{nullptr, nullptr, nullptr, nullptr, 0, 0};
+  auto fsc = [&](int step, int pair) { return scaled ? RankScale{maxf, lnf, lnf, scal + MS_REF, step, pair} : none; };
+  auto bsc = [&](int step, int pair) { return scaled ? RankScale{maxb, lnb, lnf, scal + MS_REF, step, pair} : none; };
+  auto state_of_rank = [&](int s) { return mo.seq[s - 1]; };
+  auto nosplit = [](int) { return false; };
+  // one launch of ranks sa..sb of a chain from the rank in `in`
+  auto launch = [&](int sa, int sb, const T* in, int t, unsigned* maxv, double* lnv, const T* const* outs,
+                    const T* const* pairs, const int* pidx, bool connect) {
+    B3Args<T> a{};
+    a.in = in;
+    a.g = gp(t);
+    a.nr = sb - sa + 1;
+    for (int k = 0; k < a.nr; k++) {
+      a.out[k] = const_cast<T*>(outs[k]);
+      if (connect) {
+        a.pair[k] = pairs[k];
+        a.pidx[k] = pidx[k];
+      }
+    }
+    a.acc = connect ? accp(t) : nullptr;
+    if (scaled) {
+      a.maxv = maxv;
+      a.lnv = lnv;
+      a.lnpair = lnf;
+      a.ref = scal + MS_REF;
+      a.gmax = d_gmax + t;
+    }
+    a.s0 = sa - 1;
+    prof.begin(connect ? 1 : 0, P, st, a.nr);
+    launch_block3<T>(L, a, false, scaled, connect, st);
+    prof.end(st);
+  };
+  // forward chain, ranks 2..plan.n; every rank kept (full store) or the
+  // checkpoints plus the last rank of each launch, alternating between two
+  // segment slots so a launch never overwrites its own input
+  launch_init_rank<T>(L, gp(mo.seq[0]), slot(0), fsc(1, 0), st);
+  {
+    int prev = 0, parity = 0;
+    chain_blocks(2, plan.n, 2, state_of_rank, nosplit, [&](int sa, int sb) {
+      const T* outs[kB3Ranks] = {};
+      int last = -1;
+      for (int s = sa; s <= sb; s++) {
+        int d = -1;
+        if (!plan.c)
+          d = s - 1;
+        else if ((s - 1) % plan.c == 0)
+          d = (s - 1) / plan.c;
+        else if (s == sb)
+          d = plan.seg(parity);
+        outs[s - sa] = d >= 0 ? slot(d) : nullptr;
+        if (s == sb) last = d;
+      }
+      launch(sa, sb, slot(prev), mo.seq[sa - 1], maxf, lnf, outs, nullptr, nullptr, false);
+      if (plan.c && last == plan.seg(parity)) parity ^= 1;
+      prev = last;
+    });
+  }
+  // slot of forward rank s for requests in non-increasing s; recomputes a
+  // checkpoint segment with the forward pass's own launch partition
+  int resident = -1;
+  auto get = [&](int s) -> int {
+    if (!plan.c) return s - 1;
+    const int j = (s - 1) / plan.c, start = 1 + j * plan.c;
+    if (s == start) return j;
+    if (resident != j) {
+      const int end = std::min(start + plan.c - 1, plan.n);
+      int prev = j;
+      chain_blocks(start + 1, end, 2, state_of_rank, nosplit, [&](int sa, int sb) {
+        const T* outs[kB3Ranks] = {};
+        for (int s2 = sa; s2 <= sb; s2++) outs[s2 - sa] = slot(plan.seg(s2 - start - 1));
+        launch(sa, sb, slot(prev), mo.seq[sa - 1], maxf, lnf, outs, nullptr, nullptr, false);
+        prev = plan.seg(sb - start - 1);
+      });
+      resident = j;
+    }
+    return plan.seg(s - start - 1);
+  };
+  auto segment = [&](int s) { return plan.c ? (s - 1) / plan.c : 0; };
+  const T* final_G = nullptr;
+  const double* lnfinal = nullptr;
+  int cur = 0;
+  if (mo.kind == 1) {
+    k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
+    // backward chain: step k holds G_inv of rank N-k+1
+    launch_init_rank<T>(L, gp(mo.seq[N - 1]), pp[0], bsc(1, N), st);
+    launch_connect<T, T>(L, pp[0], slot(get(N)), accp(mo.seq[N - 1]), bsc(1, N), 1.0, st);
+    auto rank_of = [&](int k) { return N - k + 1; };
+    chain_blocks(
+        2, N, 2, [&](int k) { return mo.seq[rank_of(k) - 1]; },
+        [&](int k) { return segment(rank_of(k)) != segment(rank_of(k - 1)); },
+        [&](int ka, int kb) {
+          const T* outs[kB3Ranks] = {};
+          const T* pairs[kB3Ranks] = {};
+          int pidx[kB3Ranks] = {};
+          for (int k = ka; k <= kb; k++) {
+            pairs[k - ka] = slot(get(rank_of(k)));
+            pidx[k - ka] = rank_of(k);
+          }
+          outs[kb - ka] = pp[cur ^ 1];
+          launch(ka, kb, pp[cur], mo.seq[rank_of(ka) - 1], maxb, lnb, outs, pairs, pidx, true);
+          cur ^= 1;
+        });
+    final_G = pp[cur];
+    lnfinal = scaled ? lnb + N : nullptr;
+  } else {
+    const int h = N / 2;
+    k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, h, 2.0);
+    launch_copy_interior<T>(L, slot(get(h)), pp[0], st);
+    if (N % 2 == 1) {  // the middle rank meets itself
+      const int s = h + 1;
+      launch_prop<T, T>(L, pp[0], gp(mo.seq[s - 1]), pp[1], nullptr, nullptr, fsc(s, 0), st);
+      cur = 1;
+      launch_connect<T, T>(L, pp[cur], pp[cur], accp(mo.seq[s - 1]), fsc(s, s), 0.5, st);
+    }
+    auto pair_of = [&](int s) { return N - s + 1; };
+    chain_blocks(
+        (N + 3) / 2, N, 2, state_of_rank,
+        [&](int s) { return segment(pair_of(s)) != segment(pair_of(s - 1)); },
+        [&](int sa, int sb) {
+          const T* outs[kB3Ranks] = {};
+          const T* pairs[kB3Ranks] = {};
+          int pidx[kB3Ranks] = {};
+          for (int s = sa; s <= sb; s++) {
+            pairs[s - sa] = slot(get(pair_of(s)));
+            pidx[s - sa] = pair_of(s);
+          }
+          outs[sb - sa] = pp[cur ^ 1];
+          launch(sa, sb, pp[cur], mo.seq[sa - 1], maxf, lnf, outs, pairs, pidx, true);
+          cur ^= 1;
+        });
+    final_G = pp[cur];
+    lnfinal = scaled ? lnf + N : nullptr;
+  }
+  int nb = launch_wsum<T>(L, final_G, d_partial, st);
+  launch_mol_finalize(mo.params(m), d_partial, nb, scal, lnfinal, d_molscal, nm, st);
+  for (int t = 0; t < ns; t++)
+    if (mo.uses[t]) launch_mol_phi<T>(L, accp(t), d_swf + (size_t)t * Mn, scal, 0, d_phi_m + ((size_t)m * ns + t) * Mn, st);
+}
+
 void System::compute_phi(int prec) {
   const DevLattice& L = lat.d;
   launch_swf(L, d_u, d_itmask, ns, d_st_itvar, d_swf, d_swf32, stream);
+  if (prec == SFX_FP32 && block3_ok(L)) launch_gmax(L, d_swf32, ns, d_gmax, stream);
   // non-solvent molecules first, the solvent closes phi_b (SF_MolList1stO.cpp:280-292)
   for (int pass = 0; pass < 2; pass++)
     for (int m = 0; m < nm; m++) {
@@ -505,5 +680,7 @@ double System::free_energy() {
 
 template void System::sweep<double, double>(int, bool);
 template void System::sweep<float, float>(int, bool);
+template void System::sweep_blocked<double>(int, bool);
+template void System::sweep_blocked<float>(int, bool);
 
 }  // namespace sfx

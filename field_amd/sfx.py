@@ -226,7 +226,7 @@ class System:
             if e.launches:
                 out.append({"kernel": _abi.KERNEL_NAMES[e.kind], "kind": e.kind,
                             "precision": "fp32" if e.precision == _abi.FP32 else "fp64",
-                            "launches": e.launches, "total_ms": e.total_ms,
+                            "launches": e.launches, "ranks": e.ranks, "total_ms": e.total_ms,
                             "avg_us": 1e3 * e.total_ms / e.launches, "bytes_per_launch": e.bytes_per_launch})
         return out
 

@@ -173,6 +173,9 @@ typedef struct sfx_solver_opts {
   int precondition;       /* NK: 1 = right-precondition GMRES with the Fourier-space Jacobian of the
                              homogeneous melt where it applies (periodic simple-cubic melt form);
                              0 = none. No counterpart in the reference (SFNewton has none). */
+  double ptc_dt0;         /* NK: > 0 globalises by pseudo-transient continuation, solving
+                             (I/dt + J) dx = -f from dt = ptc_dt0 with dt grown as ||f|| falls;
+                             0 = Newton steps with a backtracking line search */
   int verbose;
 } sfx_solver_opts;
 

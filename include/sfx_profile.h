@@ -5,9 +5,11 @@
  * callers that report achieved bandwidth, such as benchmark drivers.
  *
  * Between sfx_profile_start and sfx_profile_stop every propagator kernel of
- * the system is bracketed by HIP events on the stream it runs on. The report
- * sums the event durations per kernel kind and precision and gives the
- * algorithmic bytes one launch moves (interior sites only):
+ * the system is bracketed by HIP events on the stream it runs on. A launch
+ * advances one rank, or several (the blocked 3-D simple-cubic kernels). The
+ * report sums the event durations and ranks per kernel kind and precision and
+ * gives the algorithmic bytes of an average launch: ranks per launch x sites
+ * (interior) x the words of one site-rank:
  *   SFX_K_FWD   G_s = g A[G_{s-1}]                      3 words / site
  *   SFX_K_BWD   G_inv,s = g A[G_inv,s+1], phi += G G_inv 6 words / site
  *   SFX_K_TFWD  base + tangent forward step              6 words / site
@@ -29,8 +31,9 @@ typedef struct sfx_profile_entry {
   int kind;                 /* SFX_K_* */
   int precision;            /* SFX_FP64 / SFX_FP32 */
   long launches;
+  long ranks;               /* ranks those launches advanced */
   double total_ms;          /* summed event durations */
-  double bytes_per_launch;  /* algorithmic bytes of one launch */
+  double bytes_per_launch;  /* algorithmic bytes of an average launch */
 } sfx_profile_entry;
 
 typedef struct sfx_profile_report {

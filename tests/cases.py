@@ -119,12 +119,22 @@ def case(name):
                           [MoleculeSpec("p", "(A)4(B)6", "restricted", theta=12.0), MoleculeSpec("s", "(S)1", "solvent")],
                           stiffness=0.6)
         return lat, sysd
+    if name == "3d_sc_mirrors_blocked":
+        # large enough for the several-ranks-per-launch kernels; mirror1, mirror2 and
+        # periodic sides, a full-store copolymer, an odd half-stored homopolymer
+        lat = LatticeSpec(3, (9, 7, 6), lowerbound=("mirror1", "mirror2", "periodic"),
+                          upperbound=("mirror2", "mirror1", "periodic"))
+        st = ("A", "B", "S")
+        sysd = SystemSpec(st, _chi(st, [("A", "B", 0.8), ("A", "S", 0.4), ("B", "S", 0.6)]),
+                          [MoleculeSpec("p", "(A)3(B)6(A)2", "restricted", theta=40.0),
+                           MoleculeSpec("h", "(B)9", "free", phibulk=0.1), MoleculeSpec("s", "(S)1", "solvent")])
+        return lat, sysd
     raise KeyError(name)
 
 
 NAMES = ["1d_flat_adsorption", "1d_flat_odd_mirror2", "1d_cylinder", "1d_sphere", "2d_flat", "2d_cylinder",
          "3d_sc_melt", "3d_fcc_restricted", "3d_hex_three_molecules", "3d_sc_order2_homopolymer",
-         "3d_sc_order2_copolymer", "1d_order2_adsorption", "2d_order2_copolymer"]
+         "3d_sc_order2_copolymer", "1d_order2_adsorption", "2d_order2_copolymer", "3d_sc_mirrors_blocked"]
 
 
 def start(lat, sysd, seed=11, sigma=0.2):

@@ -41,6 +41,38 @@ def test_abi_version_and_struct_sizes():
     assert ctypes.sizeof(_abi.MoleculeDesc) == 40
 
 
+def test_ctypes_layouts_match_the_c_headers(tmp_path):
+    """Every ctypes mirror of an ABI struct has the size and field offsets the
+    C compiler gives the declaration in include/*.h."""
+    import subprocess
+
+    from field_amd import _abi
+
+    mirrors = {"sfx_lattice_desc": _abi.LatticeDesc, "sfx_molecule_desc": _abi.MoleculeDesc,
+               "sfx_system_desc": _abi.SystemDesc, "sfx_options": _abi.Options,
+               "sfx_solver_opts": _abi.SolverOpts, "sfx_solve_stats": _abi.SolveStats,
+               "sfx_adsorption_desc": _abi.AdsorptionDesc, "sfx_profile_entry": _abi.ProfileEntry,
+               "sfx_profile_report": _abi.ProfileReport}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sfx.h"', '#include "sfx_profile.h"',
+             "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            cf = "lambda" if fname == "lambda_" else fname
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {cf}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
 def test_errors_come_back_as_codes():
     """Invalid descriptors are rejected before any device work."""
     import pytest
