@@ -125,3 +125,20 @@ def adsorption_problem(p):
     """(lattice, system, zero start) of one batch_1d-style problem dict."""
     return adsorption_1d(p["n_layers"], p["N"], phibulk=p["phibulk"], chi_as=p["chi_as"],
                          chi_s=p["chi_s"], lam=p.get("lambda_", 1.0 / 6.0))
+
+
+def semiflexible_3d(n=128, N=200, phi=0.3, chi=0.6, stiffness_energy=1.0, sigma=0.1, seed=5):
+    """Second-order homopolymer A_N (restricted, theta = phi V) in a monomeric
+    solvent on a periodic n^3 simple-cubic lattice. The straight-bond weight is
+    S = 1/(1 + 4 exp(-eps)) for a bending energy eps per right angle (kT), so
+    each of the four perpendicular directions has (1 - S)/4 and reversal 0."""
+    lat = LatticeSpec(3, (n, n, n), order=2, lowerbound=("periodic",) * 3, upperbound=("periodic",) * 3)
+    S = 1.0 / (1.0 + 4.0 * math.exp(-stiffness_energy))
+    sysd = SystemSpec(
+        states=("A", "S"),
+        chi=np.array([[0.0, chi], [chi, 0.0]]),
+        molecules=[MoleculeSpec("pol", f"(A){N}", "restricted", theta=phi * n ** 3),
+                   MoleculeSpec("sol", "(S)1", "solvent")],
+        stiffness=S,
+    )
+    return lat, sysd, normal(seed, sysd.nvar(lat), sigma)

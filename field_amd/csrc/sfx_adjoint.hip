@@ -499,29 +499,28 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
       prev = last;
     });
   }
-  int resident = -1;
+  int resident[2] = {-1, -1};
   auto get = [&](int s) -> int {
     if (!plan.c) return s - 1;
-    const int j = (s - 1) / plan.c, start = 1 + j * plan.c;
+    const int j = (s - 1) / plan.c, start = 1 + j * plan.c, set = j & 1;
     if (s == start) return j;
-    if (resident != j) {
+    if (resident[set] != j) {
       const int end = std::min(start + plan.c - 1, plan.n);
       int prev = j;
       chain_blocks(start + 1, end, 2, state_of_rank, nosplit, [&](int sa, int sb) {
         float* outs[kB3Ranks] = {};
         float* douts[kB3Ranks] = {};
         for (int s2 = sa; s2 <= sb; s2++) {
-          outs[s2 - sa] = slot(plan.seg(s2 - start - 1));
-          douts[s2 - sa] = dslot(plan.seg(s2 - start - 1));
+          outs[s2 - sa] = slot(plan.seg_in(set, s2 - start - 1));
+          douts[s2 - sa] = dslot(plan.seg_in(set, s2 - start - 1));
         }
         launch(sa, sb, prev, false, mo.seq[sa - 1], maxf, lnf, outs, douts, nullptr, nullptr, false);
-        prev = plan.seg(sb - start - 1);
+        prev = plan.seg_in(set, sb - start - 1);
       });
-      resident = j;
+      resident[set] = j;
     }
-    return plan.seg(s - start - 1);
+    return plan.seg_in(set, s - start - 1);
   };
-  auto segment = [&](int s) { return plan.c ? (s - 1) / plan.c : 0; };
   int cur = 0;
   const float* fin = nullptr;
   const float* dfin = nullptr;
@@ -535,7 +534,7 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
     auto rank_of = [&](int k) { return N - k + 1; };
     chain_blocks(
         2, N, 2, [&](int k) { return mo.seq[rank_of(k) - 1]; },
-        [&](int k) { return segment(rank_of(k)) != segment(rank_of(k - 1)); },
+        nosplit,
         [&](int ka, int kb) {
           float* outs[kB3Ranks] = {};
           float* douts[kB3Ranks] = {};
@@ -564,7 +563,7 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
     }
     auto pair_of = [&](int s) { return N - s + 1; };
     chain_blocks(
-        (N + 3) / 2, N, 2, state_of_rank, [&](int s) { return segment(pair_of(s)) != segment(pair_of(s - 1)); },
+        (N + 3) / 2, N, 2, state_of_rank, nosplit,
         [&](int sa, int sb) {
           float* outs[kB3Ranks] = {};
           float* douts[kB3Ranks] = {};

@@ -9,12 +9,13 @@
 // on the tile widened by nr - k, ping-ponging between two LDS buffers:
 //   G_k(r) = g(r) (1/6) sum_6 G_{k-1}(r + e)            (Lat3D1stO.cpp:53-260)
 //   dG_k(r) = g(r) ((1/6) sum_6 dG_{k-1} - v(r) (1/6) sum_6 G_{k-1})
-// Tile sites of every rank go to the store and/or the composition
-// acc += G_k G_pair c_k (ConnectG, Lat3D1stO.cpp:299-312; tangent
-// dacc += (dG_k G_pair + G_k dG_pair) c_k), accumulated in LDS and added to
-// HBM once per launch. Each thread walks one (y, z) column along x and keeps
-// the x-neighbours in registers; consecutive threads take consecutive z, so
-// LDS accesses are conflict-free and HBM accesses coalesce along z.
+// Every thread owns one tile site: it stores that site's value of every rank,
+// composes acc += G_k G_pair c_k (ConnectG, Lat3D1stO.cpp:299-312; tangent
+// dacc += (dG_k G_pair + G_k dG_pair) c_k) in a register with the partner
+// values prefetched at launch start, and adds it to HBM once. The widened
+// ranks are computed by (y, z) columns, several threads per column along x;
+// consecutive threads take consecutive z, so LDS accesses are conflict-free
+// and HBM accesses coalesce along z.
 //
 // FP32 scaling: the input is divided by max(rank s0) and every rank by the
 // lattice maximum of g, so no rank exceeds 1 and at most nr ranks decay
@@ -30,7 +31,8 @@ constexpr int kRX = kTX + 2 * kH, kRY = kTY + 2 * kH, kRZ = kTZ + 2 * kH;
 constexpr int kRYZ = kRY * kRZ;
 constexpr int kR = kRX * kRYZ;  // region sites
 constexpr int kT = kTX * kTY * kTZ;
-constexpr int kB3 = 256;
+constexpr int kB3 = 1024;  // threads per workgroup, one per tile site
+static_assert(kB3 == kT, "a thread owns one tile site");
 
 bool block3_ok(const DevLattice& L) {
   return L.grad == 3 && L.kind == ST_3D_SC && L.order == 1 && L.nx > kH && L.ny > kH && L.nz > kH;
@@ -66,7 +68,6 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
   T* D0 = TANGENT ? gl + kR : nullptr;
   T* D1 = TANGENT ? D0 + kR : nullptr;
   T* vl = TANGENT ? D1 + kR : nullptr;
-  T* accT = (TANGENT ? vl : gl) + kR;
   __shared__ float red[kB3 / 64];
 
   const int t = threadIdx.x;
@@ -78,6 +79,20 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
   const int bx = b / nby;
   const int x0 = 1 + bx * kTX, y0 = 1 + by * kTY, z0 = 1 + bz * kTZ;  // tile origin (interior coords)
   const int nr = a.nr;
+
+  // the tile site this thread owns (kB3 == kT): z fastest, so HBM accesses coalesce
+  const int oZ = t % kTZ, oY = (t / kTZ) % kTY, oX = t / (kTZ * kTY);
+  const bool owner = x0 + oX <= L.nx && y0 + oY <= L.ny && z0 + oZ <= L.nz;
+  const long long ogi = owner ? sidx(L, x0 + oX, y0 + oY, z0 + oZ) : 0;
+  const int oidx = (kH + oX) * kRYZ + (kH + oY) * kRZ + (kH + oZ);
+  T pr[kB3Ranks], dpr[kB3Ranks];
+  if constexpr (CONNECT) {
+#pragma unroll
+    for (int k = 0; k < kB3Ranks; k++) {
+      pr[k] = (owner && k < nr) ? a.pair[k][ogi] : T(0);
+      if constexpr (TANGENT) dpr[k] = (owner && k < nr) ? a.dpair[k][ogi] : T(0);
+    }
+  }
 
   // scale factors of this launch
   T in_scale = T(1), ginv = T(1);
@@ -111,8 +126,6 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
         vl[li] = a.v[gi];
       }
     }
-    if constexpr (CONNECT)
-      for (int i = t; i < kT; i += kB3) accT[i] = T(0);
   }
   __syncthreads();
 
@@ -120,66 +133,54 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
   T* dst = G1;
   T* dsrc = D0;
   T* ddst = D1;
-  T mloc = T(0);
+  T acc = T(0), mloc = T(0);
   const T sixth = T(1.0 / 6.0);
   for (int k = 1; k <= nr; k++) {
+    // rank k on the tile widened by e = nr - k: threads take (y, z) columns,
+    // several threads per column split it along x
     const int e = nr - k;
     const int lo = kH - e;
     const int bxn = kTX + 2 * e, byn = kTY + 2 * e, bzn = kTZ + 2 * e;
     const int ncol = byn * bzn;
-    T* outk = a.out[k - 1];
-    T* doutk = TANGENT ? a.dout[k - 1] : nullptr;
-    const T* pk = CONNECT ? a.pair[k - 1] : nullptr;
-    const T* dpk = (CONNECT && TANGENT) ? a.dpair[k - 1] : nullptr;
-    T cfac = T(1);
-    if constexpr (SCALED && CONNECT) cfac = (T)exp(ln_base + k * lng + a.lnpair[a.pidx[k - 1]] - *a.ref);
-    for (int c = t; c < ncol; c += kB3) {
-      const int Y = lo + c / bzn, Z = lo + c % bzn;
-      const int base = Y * kRZ + Z;
-      const bool yz_tile = Y >= kH && Y < kH + kTY && Z >= kH && Z < kH + kTZ && y0 + Y - kH <= L.ny &&
-                           z0 + Z - kH <= L.nz;
-      int idx = lo * kRYZ + base;
-      T xm = src[idx - kRYZ], xc = src[idx];
-      T dxm = T(0), dxc = T(0);
-      if constexpr (TANGENT) {
-        dxm = dsrc[idx - kRYZ];
-        dxc = dsrc[idx];
-      }
-      for (int X = lo; X < lo + bxn; X++, idx += kRYZ) {
-        const T xp = src[idx + kRYZ];
-        const T s = (xm + xp + src[idx - kRZ] + src[idx + kRZ] + src[idx - 1] + src[idx + 1]) * sixth;
+    const int per = kB3 / ncol > 0 ? kB3 / ncol : 1;
+    if (t < per * ncol) {
+      const int c = t % ncol, xs = t / ncol;
+      const int base = (lo + c / bzn) * kRZ + (lo + c % bzn);
+      for (int X = lo + xs; X < lo + bxn; X += per) {
+        const int idx = X * kRYZ + base;
+        const T s = (src[idx - kRYZ] + src[idx + kRYZ] + src[idx - kRZ] + src[idx + kRZ] + src[idx - 1] +
+                     src[idx + 1]) * sixth;
         const T gv = gl[idx] * ginv;
-        const T val = gv * s;
-        dst[idx] = val;
-        T dval = T(0);
+        dst[idx] = gv * s;
         if constexpr (TANGENT) {
-          const T dxp = dsrc[idx + kRYZ];
-          const T ds = (dxm + dxp + dsrc[idx - kRZ] + dsrc[idx + kRZ] + dsrc[idx - 1] + dsrc[idx + 1]) * sixth;
-          dval = gv * (ds - vl[idx] * s);
-          ddst[idx] = dval;
-          dxm = dxc;
-          dxc = dxp;
-        }
-        xm = xc;
-        xc = xp;
-        if (yz_tile && X >= kH && X < kH + kTX && x0 + X - kH <= L.nx) {
-          const long long gi = sidx(L, x0 + X - kH, y0 + Y - kH, z0 + Z - kH);
-          if (outk) outk[gi] = val;
-          if constexpr (TANGENT) {
-            if (doutk) doutk[gi] = dval;
-          }
-          if constexpr (CONNECT) {
-            const int ti = ((X - kH) * kTY + (Y - kH)) * kTZ + (Z - kH);
-            if constexpr (TANGENT)
-              accT[ti] += (dval * pk[gi] + val * dpk[gi]) * cfac;
-            else
-              accT[ti] += val * pk[gi] * cfac;
-          }
-          if (k == nr) mloc = val > mloc ? val : mloc;
+          const T ds = (dsrc[idx - kRYZ] + dsrc[idx + kRYZ] + dsrc[idx - kRZ] + dsrc[idx + kRZ] + dsrc[idx - 1] +
+                        dsrc[idx + 1]) * sixth;
+          ddst[idx] = gv * (ds - vl[idx] * s);
         }
       }
     }
     __syncthreads();
+    // the owned tile site of rank k: store, compose, track the last rank's max
+    if (owner) {
+      const T val = dst[oidx];
+      T dval = T(0);
+      if constexpr (TANGENT) dval = ddst[oidx];
+      T* outk = a.out[k - 1];
+      if (outk) outk[ogi] = val;
+      if constexpr (TANGENT) {
+        T* doutk = a.dout[k - 1];
+        if (doutk) doutk[ogi] = dval;
+      }
+      if constexpr (CONNECT) {
+        T cfac = T(1);
+        if constexpr (SCALED) cfac = (T)exp(ln_base + k * lng + a.lnpair[a.pidx[k - 1]] - *a.ref);
+        if constexpr (TANGENT)
+          acc += (dval * pr[k - 1] + val * dpr[k - 1]) * cfac;
+        else
+          acc += val * pr[k - 1] * cfac;
+      }
+      if (k == nr) mloc = val;
+    }
     T* tmp = src;
     src = dst;
     dst = tmp;
@@ -191,11 +192,7 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
   }
 
   if constexpr (CONNECT) {
-    for (int i = t; i < kT; i += kB3) {
-      const int Z = i % kTZ, r = i / kTZ, Y = r % kTY, X = r / kTY;
-      const int x = x0 + X, y = y0 + Y, z = z0 + Z;
-      if (x <= L.nx && y <= L.ny && z <= L.nz) a.acc[sidx(L, x, y, z)] += accT[i];
-    }
+    if (owner) a.acc[ogi] += acc;
   }
   if constexpr (SCALED) {
     const T m = block_max_T<T>(mloc, red);
@@ -209,7 +206,7 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
 
 template <typename T, bool TANGENT>
 static size_t b3_lds() {
-  return ((TANGENT ? 6 : 3) * (size_t)kR + kT) * sizeof(T);
+  return (TANGENT ? 6 : 3) * (size_t)kR * sizeof(T);
 }
 
 template <typename T, bool TG, bool SC, bool CN>
@@ -251,27 +248,32 @@ void launch_block3(const DevLattice& L, const B3Args<T>& a, bool tangent, bool s
   SFX_HIP(hipGetLastError());
 }
 
-// one block per state: max of the FP32 SWF over the interior
+// max of each state's FP32 SWF over the interior: 64 workgroups per state,
+// one atomic each on the state's word (zeroed first)
+__global__ void k_gmax_zero(float* gmax, int ns) {
+  if ((int)threadIdx.x < ns) gmax[threadIdx.x] = 0.f;
+}
 __global__ void k_gmax(DevLattice L, const float* __restrict__ swf32, float* __restrict__ gmax) {
-  const float* g = swf32 + (long long)blockIdx.x * L.M;
+  const float* g = swf32 + (long long)blockIdx.y * L.M;
   float m = 0.f;
-  for (long long q = threadIdx.x; q < L.V; q += blockDim.x) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < L.V; q += (long long)gridDim.x * blockDim.x) {
     int x, y, z;
     interior_coords(L, q, x, y, z);
     m = fmaxf(m, g[sidx(L, x, y, z)]);
   }
-  __shared__ float red[16];
+  __shared__ float red[4];
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmaxf(m, red[i]);
-    gmax[blockIdx.x] = m;
+    m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    atomicMax(reinterpret_cast<unsigned*>(gmax) + blockIdx.y, __float_as_uint(m));
   }
 }
 
 void launch_gmax(const DevLattice& L, const float* swf32, int ns, float* gmax, hipStream_t st) {
-  k_gmax<<<ns, 1024, 0, st>>>(L, swf32, gmax);
+  k_gmax_zero<<<1, 64, 0, st>>>(gmax, ns);
+  k_gmax<<<dim3(64, ns), 256, 0, st>>>(L, swf32, gmax);
   SFX_HIP(hipGetLastError());
 }
 

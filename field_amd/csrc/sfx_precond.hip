@@ -12,9 +12,9 @@
 //   du'_i      = du_i - sum_j chi_ij (a dphi_j - phi0_j dphi_tot)
 //   df_i       = -dphi_tot + du'_i - mean_k du'_k          (SF_Solve.cpp:554-590)
 // and at q = 0 the restricted normalisation keeps dphi = 0, so J0(0) is the
-// projection I - 11^T/n. The preconditioner applies (sigma I + J0(q))^-1
-// (sigma = 1/dt of pseudo-transient continuation, 0 for plain Newton; the
-// pseudo-inverse at q = 0 when sigma = 0): pack the residual's interior into
+// projection I - 11^T/n. The preconditioner applies (sigma I + J0(q))^+
+// (sigma = 1/dt of pseudo-transient continuation, 0 for plain Newton; a
+// regularised pseudo-inverse per block): pack the residual's interior into
 // complex fields, DFT along z, y, x, solve the small block per q, inverse
 // DFT, unpack. The DFTs are
 // direct sums over a line with a twiddle table (any lattice size); one
@@ -73,60 +73,56 @@ __global__ void k_pack(DevLattice L, const double* __restrict__ r, int nv, cf* _
   out[q] = cf{(float)r[(long long)f * L.M + sidx(L, x, y, z)], 0.f};
 }
 
-// y(q) = (sigma I + J0(q))^-1 r(q), J0 stored [i][j][V]; Gauss-Jordan on the
-// real block with partial pivoting, pivots kept at least 1e-6 in magnitude
+// y(q) = (sigma I + J0(q))^+ r(q), J0 stored [i][j][V]. The block A is
+// solved in the Tikhonov form (A^T A + mu^2 I) y = A^T r with mu = 1e-6 max|A|
+// in FP64: an exact inverse for a regular block, and a pseudo-inverse for a
+// singular one (the projection J0(0); the checkerboard q = (pi,pi,pi) of an
+// even lattice, where an even chain's response cancels; sigma I + J0 where
+// J0 has the eigenvalue -sigma), which leaves null directions alone instead of
+// amplifying rounding noise along them.
 __global__ void k_apply_block(const cf* __restrict__ in, cf* __restrict__ out, const float* __restrict__ j0,
                               int nv, long long V, float sigma) {
   long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= V) return;
-  float A[SFX_MAXS][SFX_MAXS], re[SFX_MAXS], im[SFX_MAXS];
+  double A[SFX_MAXS][SFX_MAXS], B[SFX_MAXS][SFX_MAXS], re[SFX_MAXS], im[SFX_MAXS];
+  double amax = 0;
+  for (int i = 0; i < nv; i++)
+    for (int j = 0; j < nv; j++) {
+      A[i][j] = (double)j0[((long long)i * nv + j) * V + q] + (i == j ? (double)sigma : 0.0);
+      amax = fmax(amax, fabs(A[i][j]));
+    }
+  const double mu2 = (1e-6 * amax) * (1e-6 * amax) + 1e-300;
   for (int i = 0; i < nv; i++) {
-    for (int j = 0; j < nv; j++) A[i][j] = j0[((long long)i * nv + j) * V + q] + (i == j ? sigma : 0.f);
-    const cf v = in[(long long)i * V + q];
-    re[i] = v.x;
-    im[i] = v.y;
-  }
-  if (q == 0 && sigma == 0.f) {  // the projection is its own pseudo-inverse
-    float mr = 0.f, mi = 0.f;
-    for (int i = 0; i < nv; i++) {
-      mr += re[i] / nv;
-      mi += im[i] / nv;
+    double sr = 0, si = 0;
+    for (int k = 0; k < nv; k++) {
+      const cf v = in[(long long)k * V + q];
+      sr += A[k][i] * v.x;
+      si += A[k][i] * v.y;
     }
-    for (int i = 0; i < nv; i++) out[(long long)i * V + q] = cf{re[i] - mr, im[i] - mi};
-    return;
+    re[i] = sr;
+    im[i] = si;
+    for (int j = 0; j < nv; j++) {
+      double s = 0;
+      for (int k = 0; k < nv; k++) s += A[k][i] * A[k][j];
+      B[i][j] = s + (i == j ? mu2 : 0.0);
+    }
   }
+  // B is symmetric positive definite: Gauss-Jordan without pivoting
   for (int c = 0; c < nv; c++) {
-    int p = c;
-    for (int r = c + 1; r < nv; r++)
-      if (fabsf(A[r][c]) > fabsf(A[p][c])) p = r;
-    if (p != c) {
-      for (int j = 0; j < nv; j++) {
-        const float tmp = A[c][j];
-        A[c][j] = A[p][j];
-        A[p][j] = tmp;
-      }
-      float tmp = re[c];
-      re[c] = re[p];
-      re[p] = tmp;
-      tmp = im[c];
-      im[c] = im[p];
-      im[p] = tmp;
-    }
-    float d = A[c][c];
-    if (fabsf(d) < 1e-6f) d = d < 0.f ? -1e-6f : 1e-6f;
-    const float inv = 1.f / d;
-    for (int j = 0; j < nv; j++) A[c][j] *= inv;
+    const double inv = 1.0 / B[c][c];
+    for (int j = 0; j < nv; j++) B[c][j] *= inv;
     re[c] *= inv;
     im[c] *= inv;
     for (int r = 0; r < nv; r++) {
       if (r == c) continue;
-      const float f = A[r][c];
-      for (int j = 0; j < nv; j++) A[r][j] -= f * A[c][j];
+      const double f = B[r][c];
+      if (f == 0.0) continue;
+      for (int j = 0; j < nv; j++) B[r][j] -= f * B[c][j];
       re[r] -= f * re[c];
       im[r] -= f * im[c];
     }
   }
-  for (int i = 0; i < nv; i++) out[(long long)i * V + q] = cf{re[i], im[i]};
+  for (int i = 0; i < nv; i++) out[(long long)i * V + q] = cf{(float)re[i], (float)im[i]};
 }
 
 // real part / V -> [itvar][M] interior (halos and off-site entries 0)

@@ -15,13 +15,15 @@ namespace sfx {
 // (SF_Homopol1stO.cpp:49-51, 731-843): 2 sqrt(n) slots for one extra forward
 // pass.
 struct ChainPlan {
-  int n = 0, c = 0, nck = 0;
+  int n = 0, c = 0, nck = 0, sets = 1;
   // gran: c is a multiple of it, so segments start where launches of several
-  // ranks start
+  // ranks start; such launches may need two consecutive segments at once, so
+  // recomputed segments alternate between two sets of slots
   void init(int n_, bool checkpoint, int gran = 1) {
     n = n_;
     c = 0;
     nck = 0;
+    sets = gran > 1 ? 2 : 1;
     if (!checkpoint || n < 4) return;
     c = 1;
     while (c * c < n) c++;
@@ -32,8 +34,10 @@ struct ChainPlan {
     }
     nck = (n + c - 1) / c;
   }
-  int slots() const { return c ? nck + std::max(2, c - 1) : n; }
+  int seglen() const { return std::max(2, c - 1); }
+  int slots() const { return c ? nck + sets * seglen() : n; }
   int seg(int i) const { return nck + i; }
+  int seg_in(int set, int i) const { return nck + set * seglen() + i; }
 };
 
 // Forward pass over ranks 2..n; rank 1 is already in slot 0.

@@ -552,25 +552,24 @@ void System::sweep_blocked(int m, bool scaled) {
   }
   // slot of forward rank s for requests in non-increasing s; recomputes a
   // checkpoint segment with the forward pass's own launch partition
-  int resident = -1;
+  int resident[2] = {-1, -1};
   auto get = [&](int s) -> int {
     if (!plan.c) return s - 1;
-    const int j = (s - 1) / plan.c, start = 1 + j * plan.c;
+    const int j = (s - 1) / plan.c, start = 1 + j * plan.c, set = j & 1;
     if (s == start) return j;
-    if (resident != j) {
+    if (resident[set] != j) {
       const int end = std::min(start + plan.c - 1, plan.n);
       int prev = j;
       chain_blocks(start + 1, end, 2, state_of_rank, nosplit, [&](int sa, int sb) {
         const T* outs[kB3Ranks] = {};
-        for (int s2 = sa; s2 <= sb; s2++) outs[s2 - sa] = slot(plan.seg(s2 - start - 1));
+        for (int s2 = sa; s2 <= sb; s2++) outs[s2 - sa] = slot(plan.seg_in(set, s2 - start - 1));
         launch(sa, sb, slot(prev), mo.seq[sa - 1], maxf, lnf, outs, nullptr, nullptr, false);
-        prev = plan.seg(sb - start - 1);
+        prev = plan.seg_in(set, sb - start - 1);
       });
-      resident = j;
+      resident[set] = j;
     }
-    return plan.seg(s - start - 1);
+    return plan.seg_in(set, s - start - 1);
   };
-  auto segment = [&](int s) { return plan.c ? (s - 1) / plan.c : 0; };
   const T* final_G = nullptr;
   const double* lnfinal = nullptr;
   int cur = 0;
@@ -582,7 +581,7 @@ void System::sweep_blocked(int m, bool scaled) {
     auto rank_of = [&](int k) { return N - k + 1; };
     chain_blocks(
         2, N, 2, [&](int k) { return mo.seq[rank_of(k) - 1]; },
-        [&](int k) { return segment(rank_of(k)) != segment(rank_of(k - 1)); },
+        nosplit,
         [&](int ka, int kb) {
           const T* outs[kB3Ranks] = {};
           const T* pairs[kB3Ranks] = {};
@@ -610,7 +609,7 @@ void System::sweep_blocked(int m, bool scaled) {
     auto pair_of = [&](int s) { return N - s + 1; };
     chain_blocks(
         (N + 3) / 2, N, 2, state_of_rank,
-        [&](int s) { return segment(pair_of(s)) != segment(pair_of(s - 1)); },
+        nosplit,
         [&](int sa, int sb) {
           const T* outs[kB3Ranks] = {};
           const T* pairs[kB3Ranks] = {};
