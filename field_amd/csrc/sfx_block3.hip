@@ -108,22 +108,42 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
     lng = log((double)gm);
   }
 
-  // load rank s0 (and its tangent), g (and v) on the tile widened by nr
+  // load rank s0 (and its tangent), g (and v) on the tile widened by nr: all
+  // of a thread's HBM loads are issued before any LDS store waits on them
   {
+    constexpr int kIt = (kR + kB3 - 1) / kB3;
     const int lo = kH - nr, w = kTX + 2 * nr, wy = kTY + 2 * nr, wz = kTZ + 2 * nr;
     const int n = w * wy * wz;
-    for (int i = t; i < n; i += kB3) {
-      const int Z = i % wz, r = i / wz, Y = r % wy, X = r / wy;
-      const int x = extend(x0 + lo + X - kH, L.nx, L.periodic[0], L.mirror2_1[0], L.mirror2_2[0]);
-      const int y = extend(y0 + lo + Y - kH, L.ny, L.periodic[1], L.mirror2_1[1], L.mirror2_2[1]);
-      const int z = extend(z0 + lo + Z - kH, L.nz, L.periodic[2], L.mirror2_1[2], L.mirror2_2[2]);
-      const long long gi = sidx(L, x, y, z);
-      const int li = (lo + X) * kRYZ + (lo + Y) * kRZ + (lo + Z);
-      G0[li] = a.in[gi] * in_scale;
-      gl[li] = a.g[gi];
-      if constexpr (TANGENT) {
-        D0[li] = a.din[gi] * in_scale;
-        vl[li] = a.v[gi];
+    T vg[kIt], vgg[kIt], vd[kIt], vv[kIt];
+    int li[kIt];
+#pragma unroll
+    for (int j = 0; j < kIt; j++) {
+      const int i = t + j * kB3;
+      li[j] = -1;
+      if (i < n) {
+        const int Z = i % wz, r = i / wz, Y = r % wy, X = r / wy;
+        const int x = extend(x0 + lo + X - kH, L.nx, L.periodic[0], L.mirror2_1[0], L.mirror2_2[0]);
+        const int y = extend(y0 + lo + Y - kH, L.ny, L.periodic[1], L.mirror2_1[1], L.mirror2_2[1]);
+        const int z = extend(z0 + lo + Z - kH, L.nz, L.periodic[2], L.mirror2_1[2], L.mirror2_2[2]);
+        const long long gi = sidx(L, x, y, z);
+        li[j] = (lo + X) * kRYZ + (lo + Y) * kRZ + (lo + Z);
+        vg[j] = a.in[gi];
+        vgg[j] = a.g[gi];
+        if constexpr (TANGENT) {
+          vd[j] = a.din[gi];
+          vv[j] = a.v[gi];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kIt; j++) {
+      if (li[j] >= 0) {
+        G0[li[j]] = vg[j] * in_scale;
+        gl[li[j]] = vgg[j];
+        if constexpr (TANGENT) {
+          D0[li[j]] = vd[j] * in_scale;
+          vl[li[j]] = vv[j];
+        }
       }
     }
   }

@@ -119,6 +119,10 @@ def case(name):
                           [MoleculeSpec("p", "(A)4(B)6", "restricted", theta=12.0), MoleculeSpec("s", "(S)1", "solvent")],
                           stiffness=0.6)
         return lat, sysd
+    if name == "3d_sc_melt_n24":
+        # long enough for a checkpointed store smaller than the full one
+        lat, sysd, _ = configs.melt_3d(n=8, n_a=10, n_b=14, chi=0.4)
+        return lat, sysd
     if name == "3d_sc_mirrors_blocked":
         # large enough for the several-ranks-per-launch kernels; mirror1, mirror2 and
         # periodic sides, a full-store copolymer, an odd half-stored homopolymer
@@ -134,7 +138,8 @@ def case(name):
 
 NAMES = ["1d_flat_adsorption", "1d_flat_odd_mirror2", "1d_cylinder", "1d_sphere", "2d_flat", "2d_cylinder",
          "3d_sc_melt", "3d_fcc_restricted", "3d_hex_three_molecules", "3d_sc_order2_homopolymer",
-         "3d_sc_order2_copolymer", "1d_order2_adsorption", "2d_order2_copolymer", "3d_sc_mirrors_blocked"]
+         "3d_sc_order2_copolymer", "1d_order2_adsorption", "2d_order2_copolymer", "3d_sc_mirrors_blocked",
+         "3d_sc_melt_n24"]
 
 
 def start(lat, sysd, seed=11, sigma=0.2):

@@ -113,7 +113,7 @@ def test_negative_solvent_bulk_fraction(sfx, orc):
     assert np.abs(jv - fd).max() / np.abs(fd).max() < 1e-6
 
 
-@pytest.mark.parametrize("name", ["1d_flat_adsorption", "2d_cylinder", "3d_sc_melt", "1d_sphere",
+@pytest.mark.parametrize("name", ["1d_flat_adsorption", "2d_cylinder", "3d_sc_melt_n24", "1d_sphere",
                                   "3d_sc_mirrors_blocked", "1d_order2_adsorption",
                                   "3d_sc_order2_homopolymer", "3d_sc_order2_copolymer"])
 def test_checkpointed_sweeps_match_full_store(built, name, sfx):
