@@ -167,6 +167,13 @@ int sfx_free_energy(sfx_system* s, double* F) {
   });
 }
 
+int sfx_set_chi(sfx_system* s, int a, int b, double value) {
+  return guarded([&] {
+    SFX_REQUIRE(s, SFX_E_INVALID, "NULL argument");
+    s->s.set_chi(a, b, value);
+  });
+}
+
 int sfx_solver_defaults(sfx_solver_opts* o) {
   if (!o) return SFX_E_INVALID;
   std::memset(o, 0, sizeof *o);

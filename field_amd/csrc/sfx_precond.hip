@@ -142,6 +142,62 @@ __global__ void k_unpack(DevLattice L, const cf* __restrict__ in, int nv, const 
   out[q] = v;
 }
 
+// J0(q) of the homogeneous melt, one thread per wave vector q (layout
+// [i][j][V]); coef[(t*ns + t2)*maxN + d] holds sum_mol theta/(V N) times the
+// number of (t, t2) segment pairs at chain distance d, so S_tt'(q) is a
+// polynomial in a(q) evaluated by Horner's rule. chi, the iteration-variable
+// map and its representatives come from sp, so a chi change only reruns this.
+struct J0Shape {
+  int n[3];
+  int nv, maxN;
+  double phi0[SFX_MAXS];
+};
+__global__ void __launch_bounds__(kPB) k_build_j0(SysParams sp, J0Shape sh, const double* __restrict__ coef,
+                                                  float* __restrict__ j0, long long V) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= V) return;
+  const int ns = sp.ns, nv = sh.nv;
+  double J[SFX_MAXS][SFX_MAXS];
+  if (q == 0) {
+    for (int i = 0; i < nv; i++)
+      for (int j = 0; j < nv; j++) J[i][j] = (i == j ? 1.0 : 0.0) - 1.0 / nv;
+  } else {
+    const int qz = (int)(q % sh.n[2]);
+    const int qy = (int)((q / sh.n[2]) % sh.n[1]);
+    const int qx = (int)(q / ((long long)sh.n[1] * sh.n[2]));
+    const double a = (cos(2.0 * M_PI * qx / sh.n[0]) + cos(2.0 * M_PI * qy / sh.n[1]) +
+                      cos(2.0 * M_PI * qz / sh.n[2])) / 3.0;
+    double Sq[SFX_MAXS][SFX_MAXS];
+    for (int t = 0; t < ns; t++)
+      for (int t2 = 0; t2 < ns; t2++) {
+        const double* c = coef + ((long long)t * ns + t2) * sh.maxN;
+        double acc = 0;
+        for (int d = sh.maxN - 1; d >= 0; d--) acc = acc * a + c[d];
+        Sq[t][t2] = acc;
+      }
+    for (int col = 0; col < nv; col++) {
+      double dphi[SFX_MAXS], dup[SFX_MAXS], tot = 0, mean = 0;
+      for (int t = 0; t < ns; t++) {
+        double v = 0;
+        for (int t2 = 0; t2 < ns; t2++)
+          if (sp.st_itvar[t2] == col) v -= Sq[t][t2];
+        dphi[t] = v;
+        tot += v;
+      }
+      for (int i = 0; i < nv; i++) {
+        const int t = sp.itvar_rep[i];
+        double v = i == col ? 1.0 : 0.0;
+        for (int j = 0; j < ns; j++) v -= sp.chi[t * SFX_MAXS + j] * (a * dphi[j] - sh.phi0[j] * tot);
+        dup[i] = v;
+        mean += v / nv;
+      }
+      for (int i = 0; i < nv; i++) J[i][col] = -tot + dup[i] - mean;
+    }
+  }
+  for (int i = 0; i < nv; i++)
+    for (int j = 0; j < nv; j++) j0[((long long)i * nv + j) * V + q] = (float)J[i][j];
+}
+
 bool precond_supported(const System& S) {
   const DevLattice& L = S.lat.d;
   if (L.grad != 3 || L.kind != ST_3D_SC || L.order != 1 || S.form != SFX_FORM_MELT) return false;
@@ -160,6 +216,8 @@ struct Precond {
   int n[3] = {0, 0, 0};
   DevBuf<cf> b0, b1, tw;      // tw: forward then inverse table per axis
   DevBuf<float> j0;
+  DevBuf<double> coef;        // Debye polynomial coefficients (k_build_j0)
+  J0Shape sh{};
   long long twoff[3][2] = {};
 
   void init(const System& S) {
@@ -185,64 +243,32 @@ struct Precond {
     b1.alloc((size_t)nv * V);
     // homogeneous composition and Debye coefficients per molecule and state pair
     const int ns = S.ns;
-    std::vector<double> phi0(ns, 0.0);
-    int maxN = 1;
-    for (const Molecule& mo : S.mols) maxN = std::max(maxN, mo.N);
-    std::vector<double> c((size_t)ns * ns * maxN, 0.0);  // sum over molecules of theta/(V N) * pair counts at distance d
+    sh = J0Shape{};
+    sh.n[0] = n[0];
+    sh.n[1] = n[1];
+    sh.n[2] = n[2];
+    sh.nv = nv;
+    sh.maxN = 1;
+    for (const Molecule& mo : S.mols) sh.maxN = std::max(sh.maxN, mo.N);
+    std::vector<double> c((size_t)ns * ns * sh.maxN, 0.0);
     for (const Molecule& mo : S.mols) {
       const double w = mo.theta / ((double)V * mo.N);
       for (int s = 0; s < mo.N; s++) {
-        phi0[mo.seq[s]] += w;
+        sh.phi0[mo.seq[s]] += w;
         for (int s2 = 0; s2 < mo.N; s2++)
-          c[((size_t)mo.seq[s] * ns + mo.seq[s2]) * maxN + std::abs(s - s2)] += w;
+          c[((size_t)mo.seq[s] * ns + mo.seq[s2]) * sh.maxN + std::abs(s - s2)] += w;
       }
     }
-    std::vector<std::vector<double>> chi(ns, std::vector<double>(ns));
-    for (int i = 0; i < ns; i++)
-      for (int j = 0; j < ns; j++) chi[i][j] = S.chi[(size_t)i * ns + j];
-    std::vector<float> hj((size_t)nv * nv * V);
-    std::vector<double> Sq((size_t)ns * ns), J((size_t)nv * nv), dphi(ns), dup(nv);
-    for (int qx = 0; qx < n[0]; qx++)
-      for (int qy = 0; qy < n[1]; qy++)
-        for (int qz = 0; qz < n[2]; qz++) {
-          const long long q = ((long long)qx * n[1] + qy) * n[2] + qz;
-          if (q == 0) {
-            for (int i = 0; i < nv; i++)
-              for (int j = 0; j < nv; j++) J[(size_t)i * nv + j] = (i == j ? 1.0 : 0.0) - 1.0 / nv;
-          } else {
-            const double a = (std::cos(2 * M_PI * qx / n[0]) + std::cos(2 * M_PI * qy / n[1]) +
-                              std::cos(2 * M_PI * qz / n[2])) / 3.0;
-            for (int t = 0; t < ns; t++)
-              for (int t2 = 0; t2 < ns; t2++) {
-                double acc = 0;  // Horner in a
-                for (int d = maxN - 1; d >= 0; d--) acc = acc * a + c[((size_t)t * ns + t2) * maxN + d];
-                Sq[(size_t)t * ns + t2] = acc;
-              }
-            for (int col = 0; col < nv; col++) {
-              double tot = 0;
-              for (int t = 0; t < ns; t++) {
-                double v = 0;
-                for (int t2 = 0; t2 < ns; t2++)
-                  if (S.st_itvar[t2] == col) v -= Sq[(size_t)t * ns + t2];
-                dphi[t] = v;
-                tot += v;
-              }
-              double mean = 0;
-              for (int i = 0; i < nv; i++) {
-                const int t = S.itvar_rep[i];
-                double v = i == col ? 1.0 : 0.0;
-                for (int j = 0; j < ns; j++) v -= chi[t][j] * (a * dphi[j] - phi0[j] * tot);
-                dup[i] = v;
-                mean += v / nv;
-              }
-              for (int i = 0; i < nv; i++) J[(size_t)i * nv + col] = -tot + dup[i] - mean;
-            }
-          }
-          for (int i = 0; i < nv; i++)
-            for (int j = 0; j < nv; j++) hj[((size_t)i * nv + j) * V + q] = (float)J[(size_t)i * nv + j];
-        }
-    j0.alloc(hj.size());
-    SFX_HIP(hipMemcpy(j0.p, hj.data(), hj.size() * sizeof(float), hipMemcpyHostToDevice));
+    coef.alloc(c.size());
+    SFX_HIP(hipMemcpy(coef.p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
+    j0.alloc((size_t)nv * nv * V);
+    build_j0(S);
+  }
+
+  // J0 at the system's current chi
+  void build_j0(const System& S) {
+    k_build_j0<<<grid1(V, kPB), kPB, 0, S.stream>>>(S.sp, sh, coef.p, j0.p, V);
+    SFX_HIP(hipGetLastError());
   }
 
   void dft3(const System& S, cf* a, cf* b, int dir, cf*& result) {
@@ -279,6 +305,9 @@ Precond* precond_create(const System& S) {
 }
 void precond_apply(Precond* p, const System& S, const double* r, double* out, double sigma) {
   p->apply(S, r, out, sigma);
+}
+void precond_update(Precond* p, const System& S) {
+  if (p) p->build_j0(S);
 }
 void precond_free(Precond* p) { delete p; }
 

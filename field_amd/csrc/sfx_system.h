@@ -169,6 +169,8 @@ struct System {
   ~System();
   void create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const sfx_options* opt);
   long long M() const { return lat.d.M; }
+  // replaces chi_ab = chi_ba between solves (a Vary-style parameter step)
+  void set_chi(int a, int b, double value);
 
   // x (ABI layout, host or device) -> d_u
   void load_x(const double* x, int where);
@@ -189,6 +191,7 @@ struct System {
 bool precond_supported(const System& S);
 Precond* precond_create(const System& S);
 void precond_apply(Precond* p, const System& S, const double* r, double* out, double sigma);
+void precond_update(Precond* p, const System& S);  // after a chi change
 void precond_free(Precond* p);
 
 }  // namespace sfx

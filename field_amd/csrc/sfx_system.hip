@@ -78,6 +78,45 @@ static int is_palindrome(const std::vector<int>& s) {
   return 1;
 }
 
+// chemIntRef of one molecule: half the composition-weighted contact energy
+// sum_a f_a sum_b chi_ab f_b (SF_MolStructure.cpp:412-424)
+static double chem_int_ref(const std::vector<double>& chi, int ns, const std::vector<int>& seq) {
+  const double N = (double)seq.size();
+  double ref = 0;
+  for (int a = 0; a < ns; a++) {
+    double fa = std::count(seq.begin(), seq.end(), a) / N;
+    if (fa == 0) continue;
+    double ci = 0;
+    for (int b = 0; b < ns; b++) ci += chi[a * ns + b] * std::count(seq.begin(), seq.end(), b) / N;
+    ref += fa * ci;
+  }
+  return ref / 2;
+}
+
+void System::set_chi(int a, int b, double value) {
+  SFX_REQUIRE(a >= 0 && a < ns && b >= 0 && b < ns, SFX_E_INVALID, "state index out of range");
+  SFX_REQUIRE(std::isfinite(value), SFX_E_INVALID, "chi must be finite");
+  std::vector<double> c = chi;
+  c[a * ns + b] = c[b * ns + a] = value;
+  // the iteration-variable layout (states with equal chi rows share one) must not change
+  for (int i = 0; i < ns; i++)
+    for (int j = 0; j < i; j++) {
+      if (frozen[i] || frozen[j]) continue;
+      bool eq = true, eq0 = st_itvar[i] == st_itvar[j];
+      for (int k = 0; k < ns; k++) eq &= c[i * ns + k] == c[j * ns + k];
+      SFX_REQUIRE(eq == eq0, SFX_E_INVALID,
+                  "this chi change would merge or split iteration variables; create a new system");
+    }
+  chi = c;
+  sp.chi[a * SFX_MAXS + b] = sp.chi[b * SFX_MAXS + a] = value;
+  for (int m = 0; m < nm; m++) {
+    mols[m].chem_int_ref = chem_int_ref(chi, ns, mols[m].seq);
+    sp.mol_ref[m] = mols[m].chem_int_ref;
+  }
+  precond_update(precond, *this);  // the preconditioner is the Jacobian at this chi
+  phi_valid = false;
+}
+
 void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const sfx_options* opt) {
   lattice_init(lat, ld);
   stiffness = sd.stiffness;
@@ -171,17 +210,7 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
     mo.nstore = mo.kind == 0 ? 0 : mo.kind == 2 ? mo.N / 2 : mo.N;
     maxN = std::max(maxN, mo.N);
     maxstore = std::max(maxstore, mo.nstore);
-    // chemIntRef (SF_MolStructure.cpp:412-424)
-    double ref = 0;
-    for (int a = 0; a < ns; a++) {
-      double fa = std::count(mo.seq.begin(), mo.seq.end(), a) / (double)mo.N;
-      if (fa == 0) continue;
-      double ci = 0;
-      for (int b = 0; b < ns; b++)
-        ci += chi[a * ns + b] * std::count(mo.seq.begin(), mo.seq.end(), b) / (double)mo.N;
-      ref += fa * ci;
-    }
-    mo.chem_int_ref = ref / 2;
+    mo.chem_int_ref = chem_int_ref(chi, ns, mo.seq);
   }
   if (form == SFX_FORM_SFBOX)
     SFX_REQUIRE(nsolv == 1, SFX_E_INVALID, "exactly one solvent molecule is required (SF_MoleculeList.cpp:63-82)");

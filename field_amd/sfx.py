@@ -37,6 +37,7 @@ _SIGS = {
     "sfx_get_lngn": (C.c_int, [C.c_void_p, C.c_int, PD]),
     "sfx_get_phibulk": (C.c_int, [C.c_void_p, C.c_int, PD]),
     "sfx_free_energy": (C.c_int, [C.c_void_p, PD]),
+    "sfx_set_chi": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
     "sfx_solver_defaults": (C.c_int, [C.c_void_p]),
     "sfx_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "sfx_precondition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
@@ -214,6 +215,11 @@ class System:
         check(lib().sfx_free_energy(self.h, C.byref(v)))
         return v.value
 
+    def set_chi(self, a: int, b: int, value: float):
+        """chi(a,b) = chi(b,a) = value from here on (one step of a Vary.cpp
+        sweep over a chi parameter; the caller re-solves from its last x)."""
+        check(lib().sfx_set_chi(self.h, int(a), int(b), float(value)))
+
     def profile_start(self):
         check(lib().sfx_profile_start(self.h))
 
@@ -247,6 +253,21 @@ class System:
         if rc != 0 and not (rc == _abi.SFX_E_NOTCONVERGED and not check_converged):
             raise SFError(rc, last_error())
         return x, stats
+
+    def solve_chi_sweep(self, x0, a, b, values, method="nk", check_converged=True, **kw):
+        """A `var` sweep over chi(a,b) (Vary.cpp:3-80, SF_Box.cpp:145-176):
+        for each value in turn set chi and solve from the previous step's
+        solution; the system keeps the last value. Returns (x, [stats])."""
+        x, out = x0, []
+        for i, v in enumerate(values):
+            self.set_chi(a, b, v)
+            last = i == len(values) - 1
+            x, st = self.solve(x, method=method, check_converged=check_converged and last, **kw)
+            st["chi"] = float(v)
+            out.append(st)
+            if not st["converged"]:
+                break
+        return x, out
 
 
 class Batch:

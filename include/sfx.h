@@ -155,6 +155,12 @@ int sfx_get_phi(sfx_system* s, int state, double* phi_out /* M */, int where);
 int sfx_get_lngn(sfx_system* s, int molecule, double* lngn);
 int sfx_get_phibulk(sfx_system* s, int molecule, double* phibulk);
 int sfx_free_energy(sfx_system* s, double* F);
+/* Sets chi(a,b) = chi(b,a) = value for the following calls, keeping the
+ * current iteration vector: one step of a `var` sweep over a chi parameter
+ * (Vary.cpp:3-80; SF_Box.cpp:145-176 re-solves each step from the previous
+ * solution). SFX_E_INVALID if the change would merge or split iteration
+ * variables (states with equal chi rows share one, SF_Solve.cpp:53-86). */
+int sfx_set_chi(sfx_system* s, int a, int b, double value);
 
 /* ---- solvers ----------------------------------------------------------- */
 enum { SFX_SOLVER_DIIS = 0, SFX_SOLVER_NK = 1 };

@@ -212,6 +212,47 @@ def test_lattice_primitives_match_oracle(sfx, orc):
     sfx.lib().sfx_lattice_destroy(h)
 
 
+def test_set_chi_equals_fresh_system(sfx, orc):
+    """sfx_set_chi: residual, JVP and preconditioner after a chi change are
+    those of a system created at the new chi; a change that would merge
+    iteration variables is refused."""
+    lat, sys_a, _ = configs.melt_3d(n=7, n_a=3, n_b=5, chi=0.3)
+    _, sys_b, _ = configs.melt_3d(n=7, n_a=3, n_b=5, chi=0.45)
+    Ga, Gb = sfx.System(lat, sys_a), sfx.System(lat, sys_b)
+    mask = np.tile(lat.interior_mask().reshape(-1), 2)
+    rng = np.random.default_rng(8)
+    x = rng.normal(0, 0.2, Ga.nvar) * mask
+    v = rng.normal(0, 1, Ga.nvar) * mask
+    Ga.precondition(v)  # build the preconditioner at chi = 0.3 first
+    Ga.set_chi(0, 1, 0.45)
+    fa, fb = Ga.residual(x), Gb.residual(x)
+    assert np.array_equal(fa, fb)
+    assert np.abs(fa - orc.System(lat, sys_b).residual(x)).max() < 1e-12
+    assert np.array_equal(Ga.jvp(x, v, precision="fp64"), Gb.jvp(x, v, precision="fp64"))
+    assert np.array_equal(Ga.precondition(v), Gb.precondition(v))
+    Ga.residual(x)
+    Gb.residual(x)
+    assert Ga.free_energy() == Gb.free_energy()
+    with pytest.raises(sfx.SFError):
+        Ga.set_chi(0, 1, 0.0)  # both chi rows would be zero: one iteration variable instead of two
+    assert np.array_equal(Ga.residual(x), fb)  # a refused change leaves chi alone
+
+
+def test_chi_sweep_reaches_melt_solution(sfx, orc):
+    """A two-step `var` sweep of chi_AB (0.38, then 0.4) on the melt from its
+    seeded start ends at a solution of the chi = 0.4 system (checked by the
+    oracle's residual and free energy), with the system left at chi = 0.4."""
+    lat, sysd, x0 = configs.melt_3d(n=16)
+    G = sfx.System(lat, sysd)
+    xs, sts = G.solve_chi_sweep(x0, 0, 1, [0.38, 0.4], method="nk", tolerance=1e-9, max_iter=60, ptc_dt0=1.0)
+    print("sweep", sts)
+    assert [s["chi"] for s in sts] == [0.38, 0.4] and all(s["converged"] for s in sts)
+    O = orc.System(lat, sysd)
+    assert np.linalg.norm(O.residual(xs)) < 1e-8
+    G.residual(xs)
+    assert abs(G.free_energy() - O.free_energy()) <= 1e-10 * abs(O.free_energy())
+
+
 @pytest.mark.parametrize("which", ["diblock", "triblock_three_states"])
 def test_fourier_preconditioner_inverts_homogeneous_jacobian(sfx, which):
     """At u = 0 the melt is homogeneous and its Jacobian is exactly the
