@@ -471,9 +471,8 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
     a.ref = scal + MS_REF;
     a.gmax = S.d_gmax + t;
     a.s0 = sa - 1;
-    S.prof.begin(connect ? 3 : 2, SFX_FP32, st, a.nr);
-    launch_block3<float>(L, a, true, true, connect, st);
-    S.prof.end(st);
+    const auto ev = S.prof.events(connect ? 3 : 2, SFX_FP32, a.nr);
+    launch_block3<float>(L, a, true, true, connect, st, ev.first, ev.second);
   };
   k_tinit<float><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0));
   {

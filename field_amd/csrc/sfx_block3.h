@@ -53,9 +53,10 @@ void chain_blocks(int a, int b, int g0, State state, Split split, F f) {
 // side at least kB3Ranks + 1 layers (one reflection reaches the interior)
 bool block3_ok(const DevLattice& L);
 
+// ev0/ev1 (optional): timing events the dispatch records at kernel start and end
 template <typename T>
 void launch_block3(const DevLattice& L, const B3Args<T>& a, bool tangent, bool scaled, bool connect,
-                   hipStream_t st);
+                   hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 // max over the interior of each state's FP32 SWF, into gmax[ns] (zeroed here)
 void launch_gmax(const DevLattice& L, const float* swf32, int ns, float* gmax, hipStream_t st);

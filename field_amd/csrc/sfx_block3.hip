@@ -20,6 +20,8 @@
 // FP32 scaling: the input is divided by max(rank s0) and every rank by the
 // lattice maximum of g, so no rank exceeds 1 and at most nr ranks decay
 // between renormalisations; the logs of the divisors go to lnv.
+#include <hip/hip_ext.h>
+
 #include "sfx_block3.h"
 #include "sfx_device.h"
 
@@ -230,7 +232,8 @@ static size_t b3_lds() {
 }
 
 template <typename T, bool TG, bool SC, bool CN>
-static void b3_launch(const DevLattice& L, const B3Args<T>& a, unsigned nb, hipStream_t st) {
+static void b3_launch(const DevLattice& L, const B3Args<T>& a, unsigned nb, hipStream_t st, hipEvent_t ev0,
+                      hipEvent_t ev1) {
   static bool attr = false;  // dynamic LDS above the default limit needs the attribute once
   const size_t lds = b3_lds<T, TG>();
   if (!attr) {
@@ -238,19 +241,20 @@ static void b3_launch(const DevLattice& L, const B3Args<T>& a, unsigned nb, hipS
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr = true;
   }
-  k_block3<T, TG, SC, CN><<<nb, kB3, lds, st>>>(L, a);
+  // with events: recorded by the dispatch itself at kernel start and end
+  hipExtLaunchKernelGGL(k_block3<T, TG, SC, CN>, dim3(nb), dim3(kB3), (uint32_t)lds, st, ev0, ev1, 0u, L, a);
 }
 
 template <typename T>
 void launch_block3(const DevLattice& L, const B3Args<T>& a, bool tangent, bool scaled, bool connect,
-                   hipStream_t st) {
+                   hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   SFX_REQUIRE(a.nr >= 1 && a.nr <= kB3Ranks, SFX_E_INVALID, "blocked launch of 1..4 ranks");
   SFX_REQUIRE(!(tangent && sizeof(T) == 8), SFX_E_UNSUPPORTED, "FP64 tangent sweeps run rank by rank");
   const size_t lds_base = b3_lds<T, false>(), lds_tan = b3_lds<T, true>();
   SFX_REQUIRE((lds_base <= 160 * 1024 && (sizeof(T) == 8 || lds_tan <= 160 * 1024)), SFX_E_UNSUPPORTED,
               "blocked tile exceeds LDS");
   const unsigned nb = (unsigned)(((L.nx + kTX - 1) / kTX) * ((L.ny + kTY - 1) / kTY) * ((L.nz + kTZ - 1) / kTZ));
-#define B3(TG, SC, CN) b3_launch<T, TG, SC, CN>(L, a, nb, st)
+#define B3(TG, SC, CN) b3_launch<T, TG, SC, CN>(L, a, nb, st, ev0, ev1)
   if (tangent) {
     if (scaled) {
       if (connect) B3(true, true, true); else B3(true, true, false);
@@ -297,7 +301,9 @@ void launch_gmax(const DevLattice& L, const float* swf32, int ns, float* gmax, h
   SFX_HIP(hipGetLastError());
 }
 
-template void launch_block3<float>(const DevLattice&, const B3Args<float>&, bool, bool, bool, hipStream_t);
-template void launch_block3<double>(const DevLattice&, const B3Args<double>&, bool, bool, bool, hipStream_t);
+template void launch_block3<float>(const DevLattice&, const B3Args<float>&, bool, bool, bool, hipStream_t,
+                                   hipEvent_t, hipEvent_t);
+template void launch_block3<double>(const DevLattice&, const B3Args<double>&, bool, bool, bool, hipStream_t,
+                                    hipEvent_t, hipEvent_t);
 
 }  // namespace sfx

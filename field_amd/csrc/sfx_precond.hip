@@ -16,10 +16,10 @@
 // (sigma = 1/dt of pseudo-transient continuation, 0 for plain Newton; a
 // regularised pseudo-inverse per block): pack the residual's interior into
 // complex fields, DFT along z, y, x, solve the small block per q, inverse
-// DFT, unpack. The DFTs are
-// direct sums over a line with a twiddle table (any lattice size); one
-// thread per output element, consecutive threads along the innermost
-// contiguous axis so every load is coalesced.
+// DFT, unpack. A power-of-two side is transformed by an FFT in LDS (k_fft);
+// any other side by a direct sum over the line with a twiddle table (k_dft,
+// one thread per output element, consecutive threads along the innermost
+// contiguous axis so every load is coalesced).
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -62,6 +62,58 @@ __global__ void __launch_bounds__(kPB) k_dft(const cf* __restrict__ in, cf* __re
   out[f * field_stride + o * n * inner + (long long)k * inner + i] = cf{re, im};
 }
 
+// The same transform for a power-of-two line length n (2..1024) as an FFT in
+// LDS: radix-2 Stockham auto-sort (Govindaraju et al., SC 2008), log2(n)
+// stages ping-ponging two LDS buffers, output in natural order. A workgroup
+// takes kFftPts / n consecutive lines (line t = (field, o, i)); loads and
+// stores run along i (inner > 1) or along the line (inner == 1) so they
+// coalesce. Rows are padded to n + 1 entries against LDS bank conflicts.
+constexpr int kFftPts = 1024;
+__global__ void __launch_bounds__(kPB) k_fft(const cf* __restrict__ in, cf* __restrict__ out, long long outer,
+                                             int n, long long inner, const cf* __restrict__ tw, int fields,
+                                             long long field_stride) {
+  __shared__ cf buf[2][kFftPts + kFftPts / 2];  // LB rows of n + 1 (LB * (n + 1) <= 1.5 kFftPts for n >= 2)
+  const int LB = kFftPts / n, P = n + 1, half = n >> 1;
+  const long long nlines = outer * inner, total = nlines * fields;
+  const long long t0 = (long long)blockIdx.x * LB;
+  for (int e = threadIdx.x; e < LB * n; e += kPB) {
+    const int l = inner == 1 ? e / n : e % LB;
+    const int j = inner == 1 ? e % n : e / LB;
+    const long long T = t0 + l;
+    if (T >= total) continue;
+    const long long f = T / nlines, t = T - f * nlines, o = t / inner, i = t - o * inner;
+    buf[0][l * P + j] = in[f * field_stride + (o * n + j) * inner + i];
+  }
+  int s = 0;
+  for (int Ns = 1; Ns < n; Ns <<= 1) {
+    __syncthreads();
+    const cf* src = buf[s];
+    cf* dst = buf[s ^ 1];
+    for (int b = threadIdx.x; b < LB * half; b += kPB) {
+      const int l = b / half, jb = b - l * half;
+      const cf v0 = src[l * P + jb], v1 = src[l * P + jb + half];
+      const int k = jb & (Ns - 1);
+      const cf w = tw[k * (n / (2 * Ns))];
+      const float tx = v1.x * w.x - v1.y * w.y, ty = v1.x * w.y + v1.y * w.x;
+      const int d = ((jb - k) << 1) + k;
+      dst[l * P + d] = cf{v0.x + tx, v0.y + ty};
+      dst[l * P + d + Ns] = cf{v0.x - tx, v0.y - ty};
+    }
+    s ^= 1;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < LB * n; e += kPB) {
+    const int l = inner == 1 ? e / n : e % LB;
+    const int j = inner == 1 ? e % n : e / LB;
+    const long long T = t0 + l;
+    if (T >= total) continue;
+    const long long f = T / nlines, t = T - f * nlines, o = t / inner, i = t - o * inner;
+    out[f * field_stride + (o * n + j) * inner + i] = buf[s][l * P + j];
+  }
+}
+
+static bool fft_len(int n) { return n >= 2 && n <= kFftPts && (n & (n - 1)) == 0; }
+
 // interior of the [itvar][M] residual -> complex compact fields [itvar][V]
 __global__ void k_pack(DevLattice L, const double* __restrict__ r, int nv, cf* __restrict__ out) {
   long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,11 +132,15 @@ __global__ void k_pack(DevLattice L, const double* __restrict__ r, int nv, cf* _
 // even lattice, where an even chain's response cancels; sigma I + J0 where
 // J0 has the eigenvalue -sigma), which leaves null directions alone instead of
 // amplifying rounding noise along them.
-__global__ void k_apply_block(const cf* __restrict__ in, cf* __restrict__ out, const float* __restrict__ j0,
-                              int nv, long long V, float sigma) {
+// NV (the number of iteration variables) is a template argument so the small
+// matrices live in registers.
+template <int NV>
+__global__ void __launch_bounds__(kPB) k_apply_block(const cf* __restrict__ in, cf* __restrict__ out,
+                                                     const float* __restrict__ j0, long long V, float sigma) {
+  constexpr int nv = NV;
   long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= V) return;
-  double A[SFX_MAXS][SFX_MAXS], B[SFX_MAXS][SFX_MAXS], re[SFX_MAXS], im[SFX_MAXS];
+  double A[NV][NV], B[NV][NV], re[NV], im[NV];
   double amax = 0;
   for (int i = 0; i < nv; i++)
     for (int j = 0; j < nv; j++) {
@@ -271,13 +327,22 @@ struct Precond {
     SFX_HIP(hipGetLastError());
   }
 
+  // one axis: FFT for power-of-two lengths, the direct sum otherwise
+  void axis(const System& S, const cf* a, cf* b, long long outer, int len, long long inner, const cf* t) {
+    if (fft_len(len)) {
+      const long long lines = outer * inner * nv, lb = kFftPts / len;
+      k_fft<<<(unsigned)((lines + lb - 1) / lb), kPB, 0, S.stream>>>(a, b, outer, len, inner, t, nv, V);
+    } else {
+      k_dft<<<grid1(V * nv, kPB), kPB, 0, S.stream>>>(a, b, outer, len, inner, t, nv, V);
+    }
+  }
+
   void dft3(const System& S, cf* a, cf* b, int dir, cf*& result) {
     // z (inner 1), y (inner nz), x (inner ny nz); ping-pong a -> b -> a -> b
     const long long nx = n[0], ny = n[1], nz = n[2];
-    const unsigned g = grid1(V * nv, kPB);
-    k_dft<<<g, kPB, 0, S.stream>>>(a, b, nx * ny, (int)nz, 1, tw.p + twoff[2][dir], nv, V);
-    k_dft<<<g, kPB, 0, S.stream>>>(b, a, nx, (int)ny, nz, tw.p + twoff[1][dir], nv, V);
-    k_dft<<<g, kPB, 0, S.stream>>>(a, b, 1, (int)nx, ny * nz, tw.p + twoff[0][dir], nv, V);
+    axis(S, a, b, nx * ny, (int)nz, 1, tw.p + twoff[2][dir]);
+    axis(S, b, a, nx, (int)ny, nz, tw.p + twoff[1][dir]);
+    axis(S, a, b, 1, (int)nx, ny * nz, tw.p + twoff[0][dir]);
     result = b;
   }
 
@@ -286,7 +351,17 @@ struct Precond {
     k_pack<<<grid1(V * nv, kPB), kPB, 0, S.stream>>>(L, r, nv, b0.p);
     cf* res = nullptr;
     dft3(S, b0.p, b1.p, 0, res);  // result in b1
-    k_apply_block<<<grid1(V, kPB), kPB, 0, S.stream>>>(b1.p, b0.p, j0.p, nv, V, (float)sigma);
+    const unsigned g = grid1(V, kPB);
+    switch (nv) {
+      case 1: k_apply_block<1><<<g, kPB, 0, S.stream>>>(b1.p, b0.p, j0.p, V, (float)sigma); break;
+      case 2: k_apply_block<2><<<g, kPB, 0, S.stream>>>(b1.p, b0.p, j0.p, V, (float)sigma); break;
+      case 3: k_apply_block<3><<<g, kPB, 0, S.stream>>>(b1.p, b0.p, j0.p, V, (float)sigma); break;
+      case 4: k_apply_block<4><<<g, kPB, 0, S.stream>>>(b1.p, b0.p, j0.p, V, (float)sigma); break;
+      case 5: k_apply_block<5><<<g, kPB, 0, S.stream>>>(b1.p, b0.p, j0.p, V, (float)sigma); break;
+      case 6: k_apply_block<6><<<g, kPB, 0, S.stream>>>(b1.p, b0.p, j0.p, V, (float)sigma); break;
+      case 7: k_apply_block<7><<<g, kPB, 0, S.stream>>>(b1.p, b0.p, j0.p, V, (float)sigma); break;
+      default: k_apply_block<8><<<g, kPB, 0, S.stream>>>(b1.p, b0.p, j0.p, V, (float)sigma); break;
+    }
     dft3(S, b0.p, b1.p, 1, res);
     k_unpack<<<grid1(L.M * nv, kPB), kPB, 0, S.stream>>>(L, b1.p, nv, S.d_itmask, out);
     SFX_HIP(hipGetLastError());

@@ -39,12 +39,21 @@ static __global__ void __launch_bounds__(kDotBlock) k_multidot(DotList dl, long 
   }
 }
 
-static __global__ void k_reduce_rows(const double* partial, int nrows, int ncols, double* out) {
-  int r = threadIdx.x;
-  if (r >= nrows) return;
+// out[r] = sum_c partial[r][c]: one workgroup per row, coalesced strided
+// loads, then a fixed-order wave and workgroup reduction
+static __global__ void __launch_bounds__(kDotBlock) k_reduce_rows(const double* partial, int ncols, double* out) {
+  __shared__ double sm[kDotBlock / 64];
+  const int r = blockIdx.x;
   double s = 0;
-  for (int c = 0; c < ncols; c++) s += partial[(long long)r * ncols + c];
-  out[r] = s;
+  for (int c = threadIdx.x; c < ncols; c += kDotBlock) s += partial[(long long)r * ncols + c];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int k = 0; k < kDotBlock / 64; k++) t += sm[k];
+    out[r] = t;
+  }
 }
 
 // y = sum_k c_k v_k  (c in device memory)
@@ -86,8 +95,9 @@ VecOps::~VecOps() {
 }
 void VecOps::dots(const DotList& dl, double* host_out) {
   SFX_REQUIRE(dl.count <= kMaxDots, SFX_E_INVALID, "too many dot products");
+  if (dl.count <= 0) return;
   k_multidot<<<nblk, kDotBlock, 0, stream>>>(dl, n, partial);
-  k_reduce_rows<<<1, 64, 0, stream>>>(partial, dl.count, nblk, dres);
+  k_reduce_rows<<<dl.count, kDotBlock, 0, stream>>>(partial, nblk, dres);
   SFX_HIP(hipMemcpyAsync(host_out, dres, sizeof(double) * dl.count, hipMemcpyDeviceToHost, stream));
   SFX_HIP(hipStreamSynchronize(stream));
 }

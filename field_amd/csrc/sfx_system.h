@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <utility>
 
 #include "sfx_kernels.h"
 
@@ -106,6 +107,8 @@ struct Profiler {
   std::vector<Mark> marks;
   void begin(int kind, int prec, hipStream_t st, int ranks = 1);
   void end(hipStream_t st);
+  // a start/stop pair for a launch that records them itself (hipExtLaunchKernelGGL)
+  std::pair<hipEvent_t, hipEvent_t> events(int kind, int prec, int ranks);
   ~Profiler();
 };
 

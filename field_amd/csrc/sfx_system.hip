@@ -316,6 +316,17 @@ void Profiler::begin(int kind, int prec, hipStream_t st, int ranks) {
   SFX_HIP(hipEventRecord(pool[used], st));
   used += 2;
 }
+std::pair<hipEvent_t, hipEvent_t> Profiler::events(int kind, int prec, int ranks) {
+  if (!on) return {nullptr, nullptr};
+  while (pool.size() < used + 2) {
+    hipEvent_t e;
+    SFX_HIP(hipEventCreate(&e));
+    pool.push_back(e);
+  }
+  marks.push_back(Mark{kind, prec, used, ranks});
+  used += 2;
+  return {pool[used - 2], pool[used - 1]};
+}
 void Profiler::end(hipStream_t st) {
   if (!on) return;
   SFX_HIP(hipEventRecord(pool[marks.back().ev + 1], st));
@@ -550,9 +561,8 @@ void System::sweep_blocked(int m, bool scaled) {
       a.gmax = d_gmax + t;
     }
     a.s0 = sa - 1;
-    prof.begin(connect ? 1 : 0, P, st, a.nr);
-    launch_block3<T>(L, a, false, scaled, connect, st);
-    prof.end(st);
+    const auto ev = prof.events(connect ? 1 : 0, P, a.nr);
+    launch_block3<T>(L, a, false, scaled, connect, st, ev.first, ev.second);
   };
   // forward chain, ranks 2..plan.n; every rank kept (full store) or the
   // checkpoints plus the last rank of each launch, alternating between two
