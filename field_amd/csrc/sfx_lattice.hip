@@ -138,8 +138,9 @@ void lattice_init(Lattice& L, const sfx_lattice_desc& d) {
   }
   D.order = d.order == 2 ? 2 : 1;
   if (D.order == 2)
-    SFX_REQUIRE(d.lattice_type == SFX_LAT_STANDARD && d.geometry == SFX_FLAT, SFX_E_UNSUPPORTED,
-                "second-order chains are built for flat standard lattices (Lat1D2ndO, Lat2D2ndO, Lat3D2ndO)");
+    SFX_REQUIRE(d.lattice_type == SFX_LAT_STANDARD, SFX_E_UNSUPPORTED,
+                "second-order chains run on standard lattices (Lat1D2ndO, Lat1DCyl2ndO, Lat1DSphere2ndO, "
+                "Lat2D2ndO, Lat2DCyl2ndO, Lat3D2ndO)");
   D.ladj = d.layer_adjustment > 0 ? d.layer_adjustment : 1.0;
   D.l1 = d.lambda;
   D.l1chi = d.lambda_chi > 0 ? d.lambda_chi : d.lambda;

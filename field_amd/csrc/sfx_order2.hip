@@ -1,6 +1,7 @@
-// sfx_order2.hip -- second-order (semiflexible) propagators on flat standard
-// lattices (Lat3D2ndO.cpp; Lat2D2ndO.cpp and Lat1D2ndO.cpp as the same six
-// components with the missing axes homogeneous), base and tangent.
+// sfx_order2.hip -- second-order (semiflexible) propagators on standard
+// lattices (Lat3D2ndO.cpp; Lat2D2ndO.cpp, Lat2DCyl2ndO.cpp, Lat1D2ndO.cpp,
+// Lat1DCyl2ndO.cpp and Lat1DSphere2ndO.cpp as the same six components with
+// the missing axes homogeneous), base and tangent.
 //
 // A propagator has six direction components stored [k][M] per rank; the bond
 // into a site points along e_k, e = {+x, +y, +z, -z, -y, -x}, so 5-k is the
@@ -11,6 +12,17 @@
 // 1276-1290); a periodic bound copies plainly. A neighbour read that leaves
 // the box is remapped to its source site and, across a mirror, takes the
 // reversed component.
+//
+// Curved lattices (Lat1DCyl2ndO.cpp, Lat1DSphere2ndO.cpp, Lat2DCyl2ndO.cpp)
+// split every radial step: with u(x) = 6 lambda_+1(x) and d(x) = 6
+// lambda_-1(x) (:22-25 of each), the part u (outward) or d (inward) of the
+// step reaches the neighbouring layer and the rest stays in the layer with
+// the radial components reversed (LReflect/UReflect, Lat1DCyl2ndO.cpp:52-62):
+//   forward, k = +x: T-row k of G(x-1), times d(x), plus T-row -x of G(x),
+//                    times 1 - d(x); k = -x likewise with G(x+1) and u(x);
+//   backward, z_+x = u G^+x(x+1) + (1-u) G^-x(x), z_-x = d G^-x(x-1) + (1-d) G^+x(x).
+// The 2-D cylinder's backward sweep reads the plain outward neighbour, z_+x =
+// G^+x(x+1), although it computes the split (Lat2DCyl2ndO.cpp:223-227).
 #include "sfx_device.h"
 #include "sfx_order2.h"
 
@@ -60,7 +72,7 @@ __device__ __forceinline__ void transfer_rows(T S, T Lw, const T in[6], T y[6]) 
 }
 
 // forward: out^k(r) = g(r) y^k(r - e_k), y = T in (y of the reversed component across a mirror)
-template <typename T, bool SCALED, bool TANGENT>
+template <typename T, bool SCALED, bool TANGENT, bool CURVED>
 __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restrict__ Gin, const T* __restrict__ dGin,
                                                 const T* __restrict__ g, const T* __restrict__ v,
                                                 T* __restrict__ Gout, T* __restrict__ dGout, T S, RankScale sc) {
@@ -82,6 +94,22 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
     long long r = sidx(L, x, y, z);
     T gv = g[r] * scale;
     T vv = TANGENT ? v[r] : T(0);
+    // curved: T-rows of the site itself (the part of a radial step that stays)
+    T ys[6], dys[6];
+    T up = T(1), dn = T(1);
+    if constexpr (CURVED) {
+      up = (T)(6.0 * L.lp[x]);
+      dn = (T)(6.0 * L.lm[x]);
+      T self[6];
+#pragma unroll
+      for (int j = 0; j < 6; j++) self[j] = Gin[(long long)j * M + r];
+      transfer_rows(S, Lw, self, ys);
+      if constexpr (TANGENT) {
+#pragma unroll
+        for (int j = 0; j < 6; j++) self[j] = dGin[(long long)j * M + r];
+        transfer_rows(S, Lw, self, dys);
+      }
+    }
 #pragma unroll
     for (int k = 0; k < 6; k++) {
       bool flip;
@@ -91,6 +119,9 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
       for (int j = 0; j < 6; j++) in[j] = nbi < 0 ? T(0) : Gin[(long long)j * M + nbi];
       transfer_rows(S, Lw, in, yk);
       T a = flip ? yk[5 - k] : yk[k];
+      const bool radial = CURVED && (k == 0 || k == 5);
+      const T f = k == 0 ? dn : up;
+      if (radial) a = f * a + (T(1) - f) * ys[5 - k];
       T o = gv * a;
       Gout[(long long)k * M + r] = o;
       m = fmaxf(m, (float)o);
@@ -100,6 +131,7 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
         for (int j = 0; j < 6; j++) din[j] = nbi < 0 ? T(0) : dGin[(long long)j * M + nbi];
         transfer_rows(S, Lw, din, dyk);
         T da = flip ? dyk[5 - k] : dyk[k];
+        if (radial) da = f * da + (T(1) - f) * dys[5 - k];
         dGout[(long long)k * M + r] = gv * (da - vv * a);
       }
     }
@@ -119,7 +151,7 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
 
 // backward: out = g T z with z_j = G^j(r + e_j) (reversed component across a
 // mirror), then acc += c sum_k out^k Gp^k (tangent: dacc)
-template <typename T, bool SCALED, bool TANGENT>
+template <typename T, bool SCALED, bool TANGENT, bool CURVED>
 __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restrict__ Gin, const T* __restrict__ dGin,
                                                 const T* __restrict__ g, const T* __restrict__ v,
                                                 T* __restrict__ Gout, T* __restrict__ dGout,
@@ -150,6 +182,18 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
       int jj = flip ? 5 - j : j;
       zz[j] = nbi < 0 ? T(0) : Gin[(long long)jj * M + nbi];
       if constexpr (TANGENT) dz[j] = nbi < 0 ? T(0) : dGin[(long long)jj * M + nbi];
+    }
+    if constexpr (CURVED) {
+      const T up = L.grad == 2 ? T(1) : (T)(6.0 * L.lp[x]);  // 2-D: plain outward read
+      const T dn = (T)(6.0 * L.lm[x]);
+      const T g0 = Gin[r], g5 = Gin[5 * M + r];
+      zz[0] = up * zz[0] + (T(1) - up) * g5;
+      zz[5] = dn * zz[5] + (T(1) - dn) * g0;
+      if constexpr (TANGENT) {
+        const T d0 = dGin[r], d5 = dGin[5 * M + r];
+        dz[0] = up * dz[0] + (T(1) - up) * d5;
+        dz[5] = dn * dz[5] + (T(1) - dn) * d0;
+      }
     }
     T o[6], d[6];
     transfer_rows(S, Lw, zz, o);
@@ -244,9 +288,11 @@ __global__ void k_wsum6(DevLattice L, const T* __restrict__ G, const T* __restri
     int x, y, z;
     interior_coords(L, q, x, y, z);
     long long r = sidx(L, x, y, z);
-    // 1-D sums every layer plainly (Lat1D2ndO.cpp ComputeLnGN); 2-D and 3-D
-    // halve mirror2 edges (Lat2D2ndO.cpp:387-439, Lat3D2ndO.cpp:996-1025)
-    double w = L.grad == 1 ? 1.0 : lngn_weight(L, x, y, z);
+    // flat 1-D sums every layer plainly (Lat1D2ndO.cpp ComputeLnGN); 2-D and 3-D
+    // halve mirror2 edges (Lat2D2ndO.cpp:387-439, Lat3D2ndO.cpp:996-1025);
+    // curved lattices weight by the layer volume (Lat1DCyl2ndO.cpp:319-330,
+    // Lat2DCyl2ndO.cpp:410-462)
+    double w = L.grad == 1 && !L.curved ? 1.0 : lngn_weight(L, x, y, z);
     for (int k = 0; k < 6; k++) {
       a += (double)G[(long long)k * L.M + r] * w;
       if (dG) b += (double)dG[(long long)k * L.M + r] * w;
@@ -281,8 +327,19 @@ void launch_prop2(const DevLattice& L, bool backward, bool tangent, const T* Gin
   unsigned nb = grid1(L.V, kB2);
   bool sc_on = sc.maxv != nullptr;
   T s = (T)S;
-#define F(SC, TG) k_prop2f<T, SC, TG><<<nb, kB2, 0, st>>>(L, Gin, dGin, g, v, Gout, dGout, s, sc)
-#define B(SC, TG) k_prop2b<T, SC, TG><<<nb, kB2, 0, st>>>(L, Gin, dGin, g, v, Gout, dGout, Gp, dGp, acc, s, sc)
+#define F2(SC, TG, CV) k_prop2f<T, SC, TG, CV><<<nb, kB2, 0, st>>>(L, Gin, dGin, g, v, Gout, dGout, s, sc)
+#define B2(SC, TG, CV) \
+  k_prop2b<T, SC, TG, CV><<<nb, kB2, 0, st>>>(L, Gin, dGin, g, v, Gout, dGout, Gp, dGp, acc, s, sc)
+#define F(SC, TG) \
+  do {                                   \
+    if (L.curved) F2(SC, TG, true);      \
+    else F2(SC, TG, false);              \
+  } while (0)
+#define B(SC, TG) \
+  do {                                   \
+    if (L.curved) B2(SC, TG, true);      \
+    else B2(SC, TG, false);              \
+  } while (0)
   if (!backward) {
     if (sc_on) {
       if (tangent) F(true, true); else F(true, false);
@@ -298,6 +355,8 @@ void launch_prop2(const DevLattice& L, bool backward, bool tangent, const T* Gin
   }
 #undef F
 #undef B
+#undef F2
+#undef B2
 }
 
 template <typename T>

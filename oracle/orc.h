@@ -42,6 +42,9 @@ int orc_ncomp2(const orc_lat* L);
 double orc_wcomp2(const orc_lat* L, int k);
 void orc_propagate2(const orc_lat* L, double* Gin, const double* g, double* Gout, double S, int backward);
 double orc_lngn2(const orc_lat* L, const double* G);
+/* tests only: 1 makes the 2-D cylinder's backward sweep split its outward
+ * radial step as the forward sweep does (the reference does not) */
+void orc_set_cyl2d_split_outward(int on);
 
 /* ---- SF system ---- */
 typedef struct orc_sys orc_sys;

@@ -73,6 +73,9 @@ def lib():
         L.orc_side.argtypes = [C.c_void_p, PD, PD]
         L.orc_subtract_boundaries.argtypes = [C.c_void_p, PD]
         L.orc_diis_coeffs.argtypes = [PD, C.c_int, PD]
+        L.orc_propagate2.argtypes = [C.c_void_p, PD, PD, PD, D, C.c_int]
+        L.orc_ncomp2.argtypes = [C.c_void_p]
+        L.orc_set_cyl2d_split_outward.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -109,6 +112,17 @@ class Lattice:
 
     def lngn(self, G):
         return lib().orc_lngn(self.h, _p(_f64(G)))
+
+    def propagate2(self, Gin, g, S, backward=False):
+        """One second-order step over the lattice's lumped components
+        (ncomp2 blocks of M); halos of the input are set first."""
+        Gin = _f64(Gin).copy()
+        out = np.zeros_like(Gin)
+        lib().orc_propagate2(self.h, _p(Gin), _p(_f64(g)), _p(out), S, int(backward))
+        return out
+
+    def ncomp2(self):
+        return lib().orc_ncomp2(self.h)
 
     def sides(self, P):
         P = _f64(P).copy()
@@ -205,6 +219,12 @@ class MPS:
         err = C.c_double(0)
         rc = lib().orc_mps_scf(self.h, _p(x), tol, max_iter, m, eta, C.byref(it), C.byref(err))
         return x, it.value, err.value, rc
+
+
+def set_cyl2d_split_outward(on):
+    """Tests only: make the 2-D cylinder's second-order backward sweep split
+    its outward radial step as the forward sweep does."""
+    lib().orc_set_cyl2d_split_outward(int(bool(on)))
 
 
 def diis_coeffs(A):

@@ -1,7 +1,8 @@
 /*
  * orc_lat2.c -- ORACLE (test infrastructure only): restatement of the
- * second-order (semiflexible) flat lattices: Lat3D2ndO.cpp (six components),
- * Lat2D2ndO.cpp (five) and Lat1D2ndO.cpp (three), each in its own lumped form.
+ * second-order (semiflexible) lattices: Lat3D2ndO.cpp (six components),
+ * Lat2D2ndO.cpp and Lat2DCyl2ndO.cpp (five), Lat1D2ndO.cpp, Lat1DCyl2ndO.cpp
+ * and Lat1DSphere2ndO.cpp (three), each in its own lumped form.
  *
  * A propagator has six direction components k (k*M + site): the bond into the
  * site points along e_k with e = {+x, +y, +z, -z, -y, -x}, so component 5-k is
@@ -190,6 +191,88 @@ static void propagate_2d(const orc_lat* L, double* Gin, const double* g, double*
   setb5(L, Gout);
 }
 
+/* ---- curved forms --------------------------------------------------------
+ * Lat1DCyl2ndO.cpp and Lat1DSphere2ndO.cpp (three components; the sphere is
+ * the same code over the sphere's tables) and Lat2DCyl2ndO.cpp (five). A
+ * radial step is split by per-layer fractions l1(x) = 6 lambda_+1(x) and
+ * l_1(x) = 6 lambda_-1(x) (Lat1DCyl2ndO.cpp:22-25, Lat2DCyl2ndO.cpp:19-23):
+ * of a bond that would cross into the neighbouring layer the part l crosses
+ * and the part 1 - l stays in the layer and continues with the reversed
+ * radial component (LReflect/UReflect, Lat1DCyl2ndO.cpp:52-62,
+ * Lat2DCyl2ndO.cpp:47-61). Halos as the flat forms. */
+
+/* Lat1DCyl2ndO.cpp:64-94 (forward, PropagateF) and :149-181 (PropagateB) */
+static void propagate_1d_curved(const orc_lat* L, double* Gin, const double* g, double* Gout, double S,
+                                int backward) {
+  long M = L->M;
+  double Lw = (1.0 - S) / 4.0;
+  setb3(L, Gin);
+  const double *a0 = Gin, *a1 = Gin + M, *a2 = Gin + 2 * M;
+  for (long z = 1; z <= L->n[0]; z++) {
+    double up = 6 * L->lp[z], dn = 6 * L->lm[z]; /* l1[z], l_1[z] */
+    double o0, o1, o2;
+    if (!backward) {
+      o0 = S * (dn * a0[z - 1] + (1 - dn) * a2[z]) + 4 * Lw * (dn * a1[z - 1] + (1 - dn) * a1[z]);
+      o1 = Lw * a0[z] + (2 * Lw + S) * a1[z] + Lw * a2[z];
+      o2 = 4 * Lw * (up * a1[z + 1] + (1 - up) * a1[z]) + S * (up * a2[z + 1] + (1 - up) * a0[z]);
+    } else {
+      double out = up * a0[z + 1] + (1 - up) * a2[z]; /* UReflect(H, gz0, 0) */
+      double in = dn * a2[z - 1] + (1 - dn) * a0[z];  /* LReflect(H, gz2, 2) */
+      o0 = S * out + 4 * Lw * a1[z];
+      o1 = Lw * out + (2 * Lw + S) * a1[z] + Lw * in;
+      o2 = S * in + 4 * Lw * a1[z];
+    }
+    Gout[z] = o0 * g[z];
+    Gout[M + z] = o1 * g[z];
+    Gout[2 * M + z] = o2 * g[z];
+  }
+  setb3(L, Gout);
+}
+
+/* Lat2DCyl2ndO.cpp:63-121 (forward) and :198-255 (backward). PropagateB
+ * computes the outward split UReflect(H,gz0,0) (:223) but then reads the
+ * plain outward neighbour gz0+jx (:224-227), so on backward sweeps only the
+ * inward radial step is split; restated as it is. */
+int orc_cyl2d_split_outward = 0; /* tests only: 1 applies the outward split on backward sweeps too */
+void orc_set_cyl2d_split_outward(int on) { orc_cyl2d_split_outward = on; }
+
+static void propagate_2d_cyl(const orc_lat* L, double* Gin, const double* g, double* Gout, double S,
+                             int backward) {
+  long M = L->M, jx = L->MY;
+  double Lw = (1.0 - S) / 4.0;
+  setb5(L, Gin);
+  const double* a[5];
+  for (int k = 0; k < 5; k++) a[k] = Gin + k * M;
+  for (int x = 1; x <= L->n[0]; x++) {
+    double up = 6 * L->lp[x], dn = 6 * L->lm[x];
+    for (int y = 1; y <= L->n[1]; y++) {
+      long i = x * jx + y;
+      double o[5];
+      if (!backward) {
+        long p = i - jx, q = i - 1, r = i + 1, u = i + jx;
+        o[0] = S * (dn * a[0][p] + (1 - dn) * a[4][i]) + Lw * (dn * a[1][p] + (1 - dn) * a[1][i]) +
+               2 * Lw * (dn * a[2][p] + (1 - dn) * a[2][i]) + Lw * (dn * a[3][p] + (1 - dn) * a[3][i]);
+        o[1] = Lw * a[0][q] + S * a[1][q] + 2 * Lw * a[2][q] + Lw * a[4][q];
+        o[2] = Lw * a[0][i] + Lw * a[1][i] + S * a[2][i] + Lw * a[3][i] + Lw * a[4][i];
+        o[3] = Lw * a[0][r] + 2 * Lw * a[2][r] + S * a[3][r] + Lw * a[4][r];
+        o[4] = Lw * (up * a[1][u] + (1 - up) * a[1][i]) + 2 * Lw * (up * a[2][u] + (1 - up) * a[2][i]) +
+               Lw * (up * a[3][u] + (1 - up) * a[3][i]) + S * (up * a[4][u] + (1 - up) * a[0][i]);
+      } else {
+        double h = dn * a[4][i - jx] + (1 - dn) * a[0][i]; /* LReflect(H, gz4, 4) */
+        double w = a[0][i + jx];                           /* gz0+jx, not UReflect's H */
+        if (orc_cyl2d_split_outward) w = up * a[0][i + jx] + (1 - up) * a[4][i];
+        o[0] = S * w + Lw * a[1][i + 1] + 2 * Lw * a[2][i] + Lw * a[3][i - 1];
+        o[1] = Lw * w + S * a[1][i + 1] + 2 * Lw * a[2][i] + Lw * h;
+        o[2] = Lw * w + Lw * a[1][i + 1] + S * a[2][i] + Lw * a[3][i - 1] + Lw * h;
+        o[3] = Lw * w + 2 * Lw * a[2][i] + S * a[3][i - 1] + Lw * h;
+        o[4] = Lw * a[1][i + 1] + 2 * Lw * a[2][i] + Lw * a[3][i - 1] + S * h;
+      }
+      for (int k = 0; k < 5; k++) Gout[k * M + i] = o[k] * g[i];
+    }
+  }
+  setb5(L, Gout);
+}
+
 int orc_ncomp2(const orc_lat* L) { return L->grad == 1 ? 3 : L->grad == 2 ? 5 : 6; }
 
 /* multiplicity of a lumped component in ConnectG and ComputeLnGN
@@ -202,11 +285,17 @@ double orc_wcomp2(const orc_lat* L, int k) {
 
 void orc_propagate2(const orc_lat* L, double* Gin, const double* g, double* Gout, double S, int backward) {
   if (L->grad == 1) {
-    propagate_1d(L, Gin, g, Gout, S, backward);
+    if (L->geom != SFX_FLAT)
+      propagate_1d_curved(L, Gin, g, Gout, S, backward);
+    else
+      propagate_1d(L, Gin, g, Gout, S, backward);
     return;
   }
   if (L->grad == 2) {
-    propagate_2d(L, Gin, g, Gout, S, backward);
+    if (L->geom == SFX_CYLINDER)
+      propagate_2d_cyl(L, Gin, g, Gout, S, backward);
+    else
+      propagate_2d(L, Gin, g, Gout, S, backward);
     return;
   }
   long M = L->M, jx = L->jx, jy = L->jy;
@@ -232,6 +321,18 @@ void orc_propagate2(const orc_lat* L, double* Gin, const double* g, double* Gout
 
 double orc_lngn2(const orc_lat* L, const double* G) {
   double value = 0;
+  if (L->grad == 1 && L->geom != SFX_FLAT) {
+    /* Lat1DCyl2ndO.cpp:319-330: layer volumes L[z], mirror2 edges halved */
+    for (int z = 1; z <= L->n[0]; z++) {
+      double s = 0;
+      for (int k = 0; k < 3; k++) s += orc_wcomp2(L, k) * G[k * L->M + z];
+      double w = L->L[z];
+      if (z == 1 && L->code1[0] == SFX_BOUND_MIRROR2) w *= 0.5;
+      if (z == L->n[0] && L->code2[0] == SFX_BOUND_MIRROR2) w *= 0.5;
+      value += w * s;
+    }
+    return log(value / 6.0);
+  }
   if (L->grad == 1) { /* plain sum over the layers (Lat1D2ndO.cpp ComputeLnGN) */
     for (int z = 1; z <= L->n[0]; z++)
       for (int k = 0; k < 3; k++) value += orc_wcomp2(L, k) * G[k * L->M + z];
@@ -246,6 +347,7 @@ double orc_lngn2(const orc_lat* L, const double* G) {
         if (x == L->n[0] && L->code2[0] == SFX_BOUND_MIRROR2) w *= 0.5;
         if (y == 1 && L->code1[1] == SFX_BOUND_MIRROR2) w *= 0.5;
         if (y == L->n[1] && L->code2[1] == SFX_BOUND_MIRROR2) w *= 0.5;
+        if (L->geom == SFX_CYLINDER) w *= L->L[x]; /* Lat2DCyl2ndO.cpp:410-462 */
         double s = 0;
         for (int k = 0; k < 5; k++) s += orc_wcomp2(L, k) * G[k * L->M + i];
         value += w * s;

@@ -119,6 +119,31 @@ def case(name):
                           [MoleculeSpec("p", "(A)4(B)6", "restricted", theta=12.0), MoleculeSpec("s", "(S)1", "solvent")],
                           stiffness=0.6)
         return lat, sysd
+    if name == "1d_cyl_order2":
+        # the axis (mirror1) at the lower bound, an adsorbing outer wall
+        lat = LatticeSpec(1, (22,), geometry="cylindrical", order=2, lowerbound=("mirror1",),
+                          upperbound=("surface",), lambda_=1 / 6)
+        st = ("A", "S", "W")
+        sysd = SystemSpec(st, _chi(st, [("A", "S", 0.5), ("A", "W", -1.0)]),
+                          [MoleculeSpec("p", "(A)14", "free", phibulk=0.02), MoleculeSpec("s", "(S)1", "solvent")],
+                          frozen={"W": _wall(lat, "x1")}, stiffness=0.4)
+        return lat, sysd
+    if name == "1d_sphere_order2":
+        lat = LatticeSpec(1, (16,), geometry="spherical", order=2, lowerbound=("surface",), upperbound=("mirror1",),
+                          lambda_=1 / 6, offset_first_layer=2.0)
+        st = ("A", "B", "S", "W")
+        sysd = SystemSpec(st, _chi(st, [("A", "B", 0.9), ("A", "S", 0.1), ("B", "S", 0.9), ("A", "W", -1.0)]),
+                          [MoleculeSpec("p", "(A)4(B)8", "free", phibulk=0.03), MoleculeSpec("s", "(S)1", "solvent")],
+                          frozen={"W": _wall(lat, "x0")}, stiffness=0.3)
+        return lat, sysd
+    if name == "2d_cyl_order2":
+        lat = LatticeSpec(2, (8, 9), geometry="cylindrical", order=2, lowerbound=("mirror1", "mirror2"),
+                          upperbound=("mirror1", "mirror1"), lambda_=1 / 6)
+        st = ("A", "B", "S")
+        sysd = SystemSpec(st, _chi(st, [("A", "B", 0.8), ("A", "S", 0.2), ("B", "S", 0.6)]),
+                          [MoleculeSpec("p", "(A)4(B)6", "restricted", theta=40.0), MoleculeSpec("s", "(S)1", "solvent")],
+                          stiffness=0.6)
+        return lat, sysd
     if name == "3d_sc_melt_n24":
         # long enough for a checkpointed store smaller than the full one
         lat, sysd, _ = configs.melt_3d(n=8, n_a=10, n_b=14, chi=0.4)
@@ -139,7 +164,7 @@ def case(name):
 NAMES = ["1d_flat_adsorption", "1d_flat_odd_mirror2", "1d_cylinder", "1d_sphere", "2d_flat", "2d_cylinder",
          "3d_sc_melt", "3d_fcc_restricted", "3d_hex_three_molecules", "3d_sc_order2_homopolymer",
          "3d_sc_order2_copolymer", "1d_order2_adsorption", "2d_order2_copolymer", "3d_sc_mirrors_blocked",
-         "3d_sc_melt_n24"]
+         "3d_sc_melt_n24", "1d_cyl_order2", "1d_sphere_order2", "2d_cyl_order2"]
 
 
 def start(lat, sysd, seed=11, sigma=0.2):

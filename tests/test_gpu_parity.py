@@ -115,7 +115,7 @@ def test_negative_solvent_bulk_fraction(sfx, orc):
 
 @pytest.mark.parametrize("name", ["1d_flat_adsorption", "2d_cylinder", "3d_sc_melt_n24", "1d_sphere",
                                   "3d_sc_mirrors_blocked", "1d_order2_adsorption",
-                                  "3d_sc_order2_homopolymer", "3d_sc_order2_copolymer"])
+                                  "3d_sc_order2_homopolymer", "3d_sc_order2_copolymer", "2d_cyl_order2"])
 def test_checkpointed_sweeps_match_full_store(built, name, sfx):
     """A memory budget below the full propagator store switches to sqrt(N)
     checkpoints with recomputation; residuals and JVPs are bit-identical to the

@@ -437,3 +437,93 @@ def test_order2_lumped_mass_balance_and_closed_form(orc, grad):
     V = int(np.prod(n))
     assert abs(S.lngn(0) - (math.log(V) - 11 * 0.2)) < 1e-12
     np.testing.assert_allclose(S.phi(0)[m], 7.0 / V, rtol=1e-12)
+
+
+# ---- curved second-order lattices ------------------------------------------
+
+def _ring_volumes(geometry, n, offset=0.0):
+    x = np.arange(1, n + 1, dtype=float) + offset
+    if geometry == "cylinder":
+        return np.pi * (x ** 2 - (x - 1) ** 2)
+    return 4 / 3 * np.pi * (x ** 3 - (x - 1) ** 3)
+
+
+def _restricted_mass(orc, lat, geometry, theta=300.0):
+    sysd = SystemSpec(("A", "S"), np.array([[0, 0.4], [0.4, 0]]),
+                      [MoleculeSpec("p", "(A)12", "restricted", theta=theta), MoleculeSpec("s", "(S)1", "solvent")],
+                      stiffness=0.5)
+    O = orc.System(lat, sysd)
+    O.residual(np.random.default_rng(1).normal(0, 0.1, O.nvar))
+    MX, MY, MZ = lat.dims
+    w = np.zeros((MX, MY, MZ))
+    Lx = _ring_volumes(geometry, lat.n_layers[0], lat.offset_first_layer)
+    if lat.gradients == 1:
+        w[1:-1, 0, 0] = Lx
+    else:
+        w[1:-1, 1:-1, 0] = Lx[:, None]
+    return (w.reshape(-1) * O.phi(0)).sum() / theta - 1
+
+
+@pytest.mark.parametrize("geometry", ["cylinder", "sphere"])
+@pytest.mark.parametrize("lower, offset", [("mirror1", 0.0), ("surface", 4.0)])
+def test_curved_order2_restricted_mass_balance(orc, geometry, lower, offset):
+    """Lat1DCyl2ndO / Lat1DSphere2ndO: the forward and backward sweeps are
+    adjoint in the layer-volume inner product, so a restricted chain's
+    segments sum to theta exactly (an index or table slip breaks it). A
+    mirror2 bound on a curved lattice does not conserve even in first order
+    (its image layer has another volume), so the bounds here are mirror1 and
+    surface."""
+    lat = LatticeSpec(1, (30,), geometry=geometry, lowerbound=(lower,), upperbound=("mirror1",), order=2,
+                      offset_first_layer=offset)
+    assert abs(_restricted_mass(orc, lat, geometry)) < 1e-12
+
+
+def test_cylinder_2d_order2_mass_balance_and_backward_quirk(orc):
+    """Lat2DCyl2ndO.cpp:223-227: PropagateB computes the outward split and then
+    reads the plain outward neighbour. With that split applied (a test switch
+    in the oracle) the 2-D cylinder conserves mass exactly, so everything else
+    in the restatement is pinned; as the reference has it, mass drifts."""
+    lat = LatticeSpec(2, (12, 10), geometry="cylinder", lowerbound=("mirror1", "mirror1"),
+                      upperbound=("mirror1", "periodic"), order=2)
+    try:
+        orc.set_cyl2d_split_outward(True)
+        assert abs(_restricted_mass(orc, lat, "cylinder")) < 1e-12
+    finally:
+        orc.set_cyl2d_split_outward(False)
+    assert abs(_restricted_mass(orc, lat, "cylinder")) > 1e-4
+
+
+@pytest.mark.parametrize("backward", [False, True])
+def test_cylinder_2d_order2_reduces_to_1d(orc, backward):
+    """An axially uniform field on the 2-D cylinder: one step of the five
+    component form equals the three component Lat1DCyl2ndO step (the +-y and
+    angular components equal the 1-D in-layer component)."""
+    n, ny = 9, 4
+    S = 0.3
+    l1 = orc.Lattice(LatticeSpec(1, (n,), geometry="cylinder", lowerbound=("mirror1",), upperbound=("mirror1",),
+                                 order=2))
+    l2 = orc.Lattice(LatticeSpec(2, (n, ny), geometry="cylinder", lowerbound=("mirror1", "periodic"),
+                                 upperbound=("mirror1", "periodic"), order=2))
+    rng = np.random.default_rng(5)
+    M1, M2 = l1.M, l2.M
+    prof = rng.uniform(0.5, 1.5, (3, n))  # radial, in-layer, inward components per ring
+    g1 = np.zeros(M1)
+    g1[1:-1] = rng.uniform(0.8, 1.2, n)
+    G1 = np.zeros(3 * M1)
+    for k in range(3):
+        G1[k * M1 + 1:k * M1 + 1 + n] = prof[k]
+    MY = ny + 2
+    g2 = np.zeros((n + 2, MY))
+    g2[1:-1, 1:-1] = g1[1:-1, None]
+    G2 = np.zeros((5, n + 2, MY))
+    for k5, k3 in enumerate([0, 1, 1, 1, 2]):
+        G2[k5, 1:-1, 1:-1] = prof[k3][:, None]
+    try:
+        orc.set_cyl2d_split_outward(True)  # the 1-D form splits both radial steps
+        out2 = l2.propagate2(G2.reshape(-1), g2.reshape(-1), S, backward).reshape(5, n + 2, MY)
+    finally:
+        orc.set_cyl2d_split_outward(False)
+    out1 = l1.propagate2(G1, g1, S, backward).reshape(3, M1)
+    for k5, k3 in enumerate([0, 1, 1, 1, 2]):
+        np.testing.assert_allclose(out2[k5, 1:-1, 1:-1], np.repeat(out1[k3, 1:-1, None], ny, axis=1),
+                                   rtol=1e-13, atol=0)
