@@ -144,9 +144,11 @@ class System:
         self.nvar, self.M = nv.value, M.value
 
     def close(self):
-        if getattr(self, "h", None):
-            lib().sfx_system_destroy(self.h)
-            self.h = None
+        # at interpreter exit the module's library handle may already be gone
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.sfx_system_destroy(h)
+        self.h = None
 
     __del__ = close
 
@@ -254,15 +256,19 @@ class System:
             raise SFError(rc, last_error())
         return x, stats
 
-    def solve_chi_sweep(self, x0, a, b, values, method="nk", check_converged=True, **kw):
+    def solve_chi_sweep(self, x0, a, b, values, method="nk", check_converged=True, continuation=None, **kw):
         """A `var` sweep over chi(a,b) (Vary.cpp:3-80, SF_Box.cpp:145-176):
         for each value in turn set chi and solve from the previous step's
-        solution; the system keeps the last value. Returns (x, [stats])."""
+        solution; the system keeps the last value. `continuation` (a dict of
+        solver options) overrides kw for the warm-started steps after the
+        first, e.g. plain Newton (ptc_dt0=0) from a neighbouring solution.
+        Returns (x, [stats])."""
         x, out = x0, []
         for i, v in enumerate(values):
             self.set_chi(a, b, v)
             last = i == len(values) - 1
-            x, st = self.solve(x, method=method, check_converged=check_converged and last, **kw)
+            opts = {**kw, **(continuation or {})} if i > 0 else kw
+            x, st = self.solve(x, method=method, check_converged=check_converged and last, **opts)
             st["chi"] = float(v)
             out.append(st)
             if not st["converged"]:
@@ -295,9 +301,10 @@ class Batch:
         self.nvar, self.offsets = nv.value, offs
 
     def close(self):
-        if getattr(self, "h", None):
-            lib().sfx_batch_destroy(self.h)
-            self.h = None
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.sfx_batch_destroy(h)
+        self.h = None
 
     __del__ = close
 
