@@ -65,7 +65,10 @@ class SystemDesc(C.Structure):
 
 
 class Options(C.Structure):
-    _fields_ = [("precision", C.c_int), ("memory_budget", C.c_size_t)]
+    _fields_ = [("precision", C.c_int), ("memory_budget", C.c_size_t), ("sweep_kernel", C.c_int)]
+
+
+SWEEP_AUTO, SWEEP_PER_RANK, SWEEP_BLOCKED = 0, 1, 2
 
 
 class SolverOpts(C.Structure):

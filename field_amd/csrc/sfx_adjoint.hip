@@ -425,8 +425,8 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
   double* scal = S.d_molscal + (size_t)m * MS_COUNT;
   double* lnf = S.d_lnf + (size_t)m * (S.maxN + 2);
   double* lnb = S.d_lnb + (size_t)m * (S.maxN + 2);
-  unsigned* maxf = S.d_maxf + (size_t)m * (S.maxN + 2) * kMaxSlots;
-  unsigned* maxb = S.d_maxb + (size_t)m * (S.maxN + 2) * kMaxSlots;
+  unsigned* maxf = S.d_maxf + (size_t)m * (S.maxN + 2) * kStepWords;
+  unsigned* maxb = S.d_maxb + (size_t)m * (S.maxN + 2) * kStepWords;
   float* store = (float*)S.d_store;
   float* dstore = (float*)W.dstore.p;
   float* pp[2] = {(float*)S.d_pp, (float*)S.d_pp + Mn};
@@ -441,8 +441,8 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
   const ChainPlan& plan = mo.plan;
   for (int t = 0; t < S.ns; t++)
     if (mo.uses[t]) zero<float>(daccp(t), Mn, st);
-  zero<unsigned>(maxf, (S.maxN + 2) * kMaxSlots, st);
-  zero<unsigned>(maxb, (S.maxN + 2) * kMaxSlots, st);
+  zero<unsigned>(maxf, (S.maxN + 2) * kStepWords, st);
+  zero<unsigned>(maxb, (S.maxN + 2) * kStepWords, st);
   auto fsc = [&](int step, int pair) { return RankScale{maxf, lnf, lnf, scal + MS_REF, step, pair}; };
   auto bsc = [&](int step, int pair) { return RankScale{maxb, lnb, lnf, scal + MS_REF, step, pair}; };
   auto state_of_rank = [&](int s) { return mo.seq[s - 1]; };
@@ -469,7 +469,7 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
     a.lnv = lnv;
     a.lnpair = lnf;
     a.ref = scal + MS_REF;
-    a.gmax = S.d_gmax + t;
+    a.gmax = S.d_gmax + (size_t)t * kStepWords;
     a.s0 = sa - 1;
     const auto ev = S.prof.events(connect ? 3 : 2, SFX_FP32, a.nr);
     launch_block3<float>(L, a, true, true, connect, st, ev.first, ev.second);
@@ -588,7 +588,7 @@ template <typename T>
 static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const T* g32or64, const double* v64) {
   if constexpr (std::is_same<T, float>::value) {
     const int kind = S.mols[m].kind;
-    if (scaled && block3_ok(S.lat.d) && (kind == 1 || kind == 2)) {
+    if (scaled && S.use_block3 && (kind == 1 || kind == 2)) {
       tsweep_blocked(S, W, m, vt, g32or64);
       return;
     }
@@ -600,8 +600,8 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
   double* scal = S.d_molscal + (size_t)m * MS_COUNT;
   double* lnf = S.d_lnf + (size_t)m * (S.maxN + 2);
   double* lnb = S.d_lnb + (size_t)m * (S.maxN + 2);
-  unsigned* maxf = S.d_maxf + (size_t)m * (S.maxN + 2) * kMaxSlots;
-  unsigned* maxb = S.d_maxb + (size_t)m * (S.maxN + 2) * kMaxSlots;
+  unsigned* maxf = S.d_maxf + (size_t)m * (S.maxN + 2) * kStepWords;
+  unsigned* maxb = S.d_maxb + (size_t)m * (S.maxN + 2) * kStepWords;
   T* store = (T*)S.d_store;
   T* dstore = (T*)W.dstore.p;
   T* pp[2] = {(T*)S.d_pp, (T*)S.d_pp + Mn};
@@ -616,8 +616,8 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
     if (mo.uses[t]) zero<T>(daccp(t), Mn, st);
   RankScale none{nullptr, nullptr, nullptr, nullptr, 0, 0};
   if (scaled) {
-    zero<unsigned>(maxf, (S.maxN + 2) * kMaxSlots, st);
-    zero<unsigned>(maxb, (S.maxN + 2) * kMaxSlots, st);
+    zero<unsigned>(maxf, (S.maxN + 2) * kStepWords, st);
+    zero<unsigned>(maxb, (S.maxN + 2) * kStepWords, st);
   }
   auto fsc = [&](int step, int pair) { return scaled ? RankScale{maxf, lnf, lnf, scal + MS_REF, step, pair} : none; };
   auto bsc = [&](int step, int pair) { return scaled ? RankScale{maxb, lnb, lnf, scal + MS_REF, step, pair} : none; };
@@ -761,7 +761,7 @@ void jvp_at_base(System& S, const double* v, double* jv, int precision) {
   bool fp32 = precision == SFX_FP32;
   float* v32 = (float*)W.v32.p;
   if (fp32) k_state_dir<float><<<grid1(Mn, kTB), kTB, 0, st>>>(Mn, S.ns, S.d_st_itvar, v, v32);
-  if (fp32 && block3_ok(L)) launch_gmax(L, S.d_swf32, S.ns, S.d_gmax, st);
+  if (fp32 && S.use_block3) launch_gmax(L, S.d_swf32, S.ns, S.d_gmax, st);
   for (int pass = 0; pass < 2; pass++)
     for (int m = 0; m < S.nm; m++) {
       bool solv = S.mols[m].freedom == SFX_SOLVENT;

@@ -28,7 +28,7 @@ struct B3Args {
   double* lnv;
   const double* lnpair;
   const double* ref;
-  const float* gmax;
+  const unsigned* gmax;        // step-max slots of the state's lattice maximum of g
   int s0;
   int pidx[kB3Ranks];
 };
@@ -58,7 +58,8 @@ template <typename T>
 void launch_block3(const DevLattice& L, const B3Args<T>& a, bool tangent, bool scaled, bool connect,
                    hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
-// max over the interior of each state's FP32 SWF, into gmax[ns] (zeroed here)
-void launch_gmax(const DevLattice& L, const float* swf32, int ns, float* gmax, hipStream_t st);
+// max over the interior of each state's FP32 SWF, into step-max slots
+// gmax + t * kStepWords (zeroed here; read with get_step_max(.., 0))
+void launch_gmax(const DevLattice& L, const float* swf32, int ns, unsigned* gmax, hipStream_t st);
 
 }  // namespace sfx

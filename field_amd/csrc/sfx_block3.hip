@@ -22,6 +22,8 @@
 // between renormalisations; the logs of the divisors go to lnv.
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+
 #include "sfx_block3.h"
 #include "sfx_device.h"
 
@@ -96,18 +98,34 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
     }
   }
 
-  // scale factors of this launch
+  // scale factors of this launch. They are uniform, so every wave evaluates
+  // each FP64 log / exp once in one lane and broadcasts it: lanes 0 and 1 the
+  // logs of the two divisors, lane k the composition factor of rank k + 1.
   T in_scale = T(1), ginv = T(1);
   double ln_base = 0.0, lng = 0.0;
+  T cfac[kB3Ranks];
+#pragma unroll
+  for (int k = 0; k < kB3Ranks; k++) cfac[k] = T(1);
   if constexpr (SCALED) {
+    const int lane = t & 63;
     float m0 = get_step_max(a.maxv, a.s0);
     if (!(m0 > 0.f)) m0 = 1.f;
-    float gm = *a.gmax;
+    float gm = get_step_max(a.gmax, 0);
     if (!(gm > 0.f)) gm = 1.f;
     in_scale = (T)(1.f / m0);
     ginv = (T)(1.f / gm);
-    ln_base = a.lnv[a.s0] + log((double)m0);
-    lng = log((double)gm);
+    const double lv = lane == 0 ? log((double)m0) : (lane == 1 ? log((double)gm) : 0.0);
+    ln_base = a.lnv[a.s0] + __shfl(lv, 0, 64);
+    lng = __shfl(lv, 1, 64);
+    if constexpr (CONNECT) {
+      int pi = 0;
+#pragma unroll
+      for (int k = 0; k < kB3Ranks; k++)
+        if (lane == k) pi = a.pidx[k];
+      const double c = lane < nr ? exp(ln_base + (lane + 1) * lng + a.lnpair[pi] - *a.ref) : 1.0;
+#pragma unroll
+      for (int k = 0; k < kB3Ranks; k++) cfac[k] = (T)__shfl(c, k, 64);
+    }
   }
 
   // load rank s0 (and its tangent), g (and v) on the tile widened by nr: all
@@ -194,12 +212,10 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
         if (doutk) doutk[ogi] = dval;
       }
       if constexpr (CONNECT) {
-        T cfac = T(1);
-        if constexpr (SCALED) cfac = (T)exp(ln_base + k * lng + a.lnpair[a.pidx[k - 1]] - *a.ref);
         if constexpr (TANGENT)
-          acc += (dval * pr[k - 1] + val * dpr[k - 1]) * cfac;
+          acc += (dval * pr[k - 1] + val * dpr[k - 1]) * cfac[k - 1];
         else
-          acc += val * pr[k - 1] * cfac;
+          acc += val * pr[k - 1] * cfac[k - 1];
       }
       if (k == nr) mloc = val;
     }
@@ -274,10 +290,11 @@ void launch_block3(const DevLattice& L, const B3Args<T>& a, bool tangent, bool s
 
 // max of each state's FP32 SWF over the interior: 64 workgroups per state,
 // one atomic each on the state's word (zeroed first)
-__global__ void k_gmax_zero(float* gmax, int ns) {
-  if ((int)threadIdx.x < ns) gmax[threadIdx.x] = 0.f;
+__global__ void k_gmax_zero(unsigned* gmax, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) gmax[i] = 0u;
 }
-__global__ void k_gmax(DevLattice L, const float* __restrict__ swf32, float* __restrict__ gmax) {
+__global__ void __launch_bounds__(256) k_gmax(DevLattice L, const float* __restrict__ swf32, unsigned* __restrict__ gmax) {
   const float* g = swf32 + (long long)blockIdx.y * L.M;
   float m = 0.f;
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < L.V; q += (long long)gridDim.x * blockDim.x) {
@@ -291,13 +308,15 @@ __global__ void k_gmax(DevLattice L, const float* __restrict__ swf32, float* __r
   __syncthreads();
   if (threadIdx.x == 0) {
     m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    atomicMax(reinterpret_cast<unsigned*>(gmax) + blockIdx.y, __float_as_uint(m));
+    put_step_max(gmax + (long long)blockIdx.y * kStepWords, 0, m);  // slot blockIdx.x mod kMaxSlots
   }
 }
 
-void launch_gmax(const DevLattice& L, const float* swf32, int ns, float* gmax, hipStream_t st) {
-  k_gmax_zero<<<1, 64, 0, st>>>(gmax, ns);
-  k_gmax<<<dim3(64, ns), 256, 0, st>>>(L, swf32, gmax);
+void launch_gmax(const DevLattice& L, const float* swf32, int ns, unsigned* gmax, hipStream_t st) {
+  const int n = ns * kStepWords;
+  k_gmax_zero<<<grid1(n, 256), 256, 0, st>>>(gmax, n);
+  const unsigned nb = (unsigned)std::min<long long>(256, std::max<long long>(1, L.V / 1024));
+  k_gmax<<<dim3(nb, ns), 256, 0, st>>>(L, swf32, gmax);
   SFX_HIP(hipGetLastError());
 }
 

@@ -244,7 +244,13 @@ int sfx_profile_stop(sfx_system* s, sfx_profile_report* rep) {
     S.sync();
     p.on = false;
     std::memset(rep, 0, sizeof *rep);
-    const double words[SFX_K_COUNT] = {3, 6, 6, 10};
+    // algorithmic words per site-rank: first order reads the previous rank and
+    // g and writes the new rank (compose adds the partner and the accumulator's
+    // read and write; the tangent doubles the propagator words and adds v);
+    // second order moves all six direction components of every propagator
+    const double words1[SFX_K_COUNT] = {3, 6, 6, 10};
+    const double words2[SFX_K_COUNT] = {13, 21, 26, 46};
+    const double* words = S.lat.d.order == 2 ? words2 : words1;
     for (int k = 0; k < SFX_K_COUNT; k++)
       for (int pr = 0; pr < 2; pr++) {
         sfx_profile_entry& e = rep->entries[k * 2 + pr];

@@ -16,18 +16,23 @@ enum { MS_LNGN = 0, MS_LNC, MS_PHIB, MS_PREF, MS_REF, MS_COUNT = 8 };
 // multiplied by 1/maxv[k-1]. Composition multiplies by
 // exp(lnv[k] + lnpair[pair] - *ref). maxv == nullptr disables scaling.
 // The maximum of a step is spread over kMaxSlots words (workgroup b writes
-// word b mod kMaxSlots) so that atomics from many workgroups do not queue on
-// one address; a reader reduces the words with one wave load.
-constexpr int kMaxSlots = 32;
+// word b mod kMaxSlots), each in a cache line of its own (kSlotStride words
+// apart): atomics on one line serialise, and a per-rank launch on a large
+// lattice has thousands of workgroups. A reader reduces the words with one
+// wave load. A step occupies kStepWords words.
+constexpr int kMaxSlots = 64;
+constexpr int kSlotStride = 32;  // 128-byte lines
+constexpr int kStepWords = kMaxSlots * kSlotStride;
 
 __device__ __forceinline__ void put_step_max(unsigned* maxv, int step, float m) {
-  atomicMax(maxv + (long long)step * kMaxSlots + (blockIdx.x & (kMaxSlots - 1)), __float_as_uint(m));
+  atomicMax(maxv + (long long)step * kStepWords + (blockIdx.x & (kMaxSlots - 1)) * kSlotStride,
+            __float_as_uint(m));
 }
 
 // called by every lane of a wave; all lanes get the maximum
 __device__ __forceinline__ float get_step_max(const unsigned* maxv, int step) {
   const int lane = threadIdx.x & 63;
-  float v = lane < kMaxSlots ? __uint_as_float(maxv[(long long)step * kMaxSlots + lane]) : 0.f;
+  float v = lane < kMaxSlots ? __uint_as_float(maxv[(long long)step * kStepWords + lane * kSlotStride]) : 0.f;
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
   return v;
 }

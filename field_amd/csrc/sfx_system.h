@@ -130,6 +130,7 @@ struct System {
   int maxN = 1, maxstore = 1;
   int store_slots = 1;     // rank slots of the propagator store (all molecules share it)
   bool checkpointed = false;
+  bool use_block3 = false;  // blocked 3-D simple-cubic sweeps (sfx_options.sweep_kernel)
   SysParams sp{};
   hipStream_t stream = nullptr;
 
@@ -157,7 +158,7 @@ struct System {
   unsigned* d_maxf = nullptr;
   unsigned* d_maxb = nullptr;
   double* d_partial = nullptr;  // reduction partials
-  float* d_gmax = nullptr;      // ns: lattice maximum of each FP32 SWF (blocked sweeps)
+  unsigned* d_gmax = nullptr;   // ns x kStepWords: lattice maximum of each FP32 SWF (blocked sweeps)
   double* d_phib_state = nullptr;
   double* d_cbulk = nullptr;
   double* d_scalar = nullptr;   // misc device scalars

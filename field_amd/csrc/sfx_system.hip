@@ -130,6 +130,11 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
   SFX_REQUIRE(sd.chi && sd.molecules, SFX_E_INVALID, "chi and molecules are required");
   precision = opt ? opt->precision : SFX_FP64;
   budget = opt ? opt->memory_budget : 0;
+  {
+    const int sk = opt ? opt->sweep_kernel : SFX_SWEEP_AUTO;
+    SFX_REQUIRE(sk >= SFX_SWEEP_AUTO && sk <= SFX_SWEEP_BLOCKED, SFX_E_INVALID, "unknown sweep_kernel");
+    use_block3 = sk != SFX_SWEEP_PER_RANK && block3_ok(lat.d);
+  }
   chi.assign(sd.chi, sd.chi + ns * ns);
   for (int i = 0; i < ns; i++)
     for (int j = 0; j < ns; j++)
@@ -261,7 +266,7 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
   checkpointed = budget != 0 && (size_t)maxstore * slot_bytes > budget;
   store_slots = 1;
   for (Molecule& mo : mols) {
-    mo.plan.init(mo.nstore, checkpointed, block3_ok(lat.d) ? kB3Ranks : 1);
+    mo.plan.init(mo.nstore, checkpointed, use_block3 ? kB3Ranks : 1);
     store_slots = std::max(store_slots, mo.plan.slots());
   }
   if (checkpointed)
@@ -272,11 +277,11 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
   d_molscal = dalloc<double>((size_t)nm * MS_COUNT);
   d_lnf = dalloc<double>((size_t)nm * (maxN + 2));
   d_lnb = dalloc<double>((size_t)nm * (maxN + 2));
-  d_maxf = dalloc<unsigned>((size_t)nm * (maxN + 2) * kMaxSlots);
-  d_maxb = dalloc<unsigned>((size_t)nm * (maxN + 2) * kMaxSlots);
+  d_maxf = dalloc<unsigned>((size_t)nm * (maxN + 2) * kStepWords);
+  d_maxb = dalloc<unsigned>((size_t)nm * (maxN + 2) * kStepWords);
   npart = (int)grid1(std::max<long long>(lat.d.M, lat.d.V), 256);
   d_partial = dalloc<double>(npart);
-  d_gmax = dalloc<float>(SFX_MAXS);
+  d_gmax = dalloc<unsigned>((size_t)SFX_MAXS * kStepWords);
   d_phib_state = dalloc<double>(SFX_MAXS);
   d_cbulk = dalloc<double>(SFX_MAXS);
   d_scalar = dalloc<double>(16);
@@ -379,8 +384,8 @@ void System::sweep(int m, bool scaled) {
   double* scal = d_molscal + (size_t)m * MS_COUNT;
   double* lnf = d_lnf + (size_t)m * (maxN + 2);
   double* lnb = d_lnb + (size_t)m * (maxN + 2);
-  unsigned* maxf = d_maxf + (size_t)m * (maxN + 2) * kMaxSlots;
-  unsigned* maxb = d_maxb + (size_t)m * (maxN + 2) * kMaxSlots;
+  unsigned* maxf = d_maxf + (size_t)m * (maxN + 2) * kStepWords;
+  unsigned* maxb = d_maxb + (size_t)m * (maxN + 2) * kStepWords;
   T* store = (T*)d_store;
   T* pp[2] = {(T*)d_pp, (T*)d_pp + Mn};
   A* acc = (A*)d_acc;
@@ -394,8 +399,8 @@ void System::sweep(int m, bool scaled) {
   for (int t = 0; t < ns; t++)
     if (mo.uses[t]) zero_bytes(accp(t), (size_t)Mn * sizeof(A), st);
   if (scaled) {
-    k_fill_u32<<<grid1((N + 2) * kMaxSlots, 64), 64, 0, st>>>(maxf, 0u, (N + 2) * kMaxSlots);
-    k_fill_u32<<<grid1((N + 2) * kMaxSlots, 64), 64, 0, st>>>(maxb, 0u, (N + 2) * kMaxSlots);
+    k_fill_u32<<<grid1((N + 2) * kStepWords, 256), 256, 0, st>>>(maxf, 0u, (N + 2) * kStepWords);
+    k_fill_u32<<<grid1((N + 2) * kStepWords, 256), 256, 0, st>>>(maxb, 0u, (N + 2) * kStepWords);
   }
   RankScale none{nullptr, nullptr, nullptr, nullptr, 0, 0};
   auto fsc = [&](int step, int pair) {
@@ -448,7 +453,7 @@ void System::sweep(int m, bool scaled) {
     return;
   }
   if constexpr (std::is_same<T, A>::value) {
-    if (block3_ok(L)) {
+    if (use_block3) {
       sweep_blocked<T>(m, scaled);
       return;
     }
@@ -519,8 +524,8 @@ void System::sweep_blocked(int m, bool scaled) {
   double* scal = d_molscal + (size_t)m * MS_COUNT;
   double* lnf = d_lnf + (size_t)m * (maxN + 2);
   double* lnb = d_lnb + (size_t)m * (maxN + 2);
-  unsigned* maxf = d_maxf + (size_t)m * (maxN + 2) * kMaxSlots;
-  unsigned* maxb = d_maxb + (size_t)m * (maxN + 2) * kMaxSlots;
+  unsigned* maxf = d_maxf + (size_t)m * (maxN + 2) * kStepWords;
+  unsigned* maxb = d_maxb + (size_t)m * (maxN + 2) * kStepWords;
   T* store = (T*)d_store;
   T* pp[2] = {(T*)d_pp, (T*)d_pp + Mn};
   T* acc = (T*)d_acc;
@@ -558,7 +563,7 @@ void System::sweep_blocked(int m, bool scaled) {
       a.lnv = lnv;
       a.lnpair = lnf;
       a.ref = scal + MS_REF;
-      a.gmax = d_gmax + t;
+      a.gmax = d_gmax + (size_t)t * kStepWords;
     }
     a.s0 = sa - 1;
     const auto ev = prof.events(connect ? 1 : 0, P, a.nr);
@@ -673,7 +678,7 @@ void System::sweep_blocked(int m, bool scaled) {
 void System::compute_phi(int prec) {
   const DevLattice& L = lat.d;
   launch_swf(L, d_u, d_itmask, ns, d_st_itvar, d_swf, d_swf32, stream);
-  if (prec == SFX_FP32 && block3_ok(L)) launch_gmax(L, d_swf32, ns, d_gmax, stream);
+  if (prec == SFX_FP32 && use_block3) launch_gmax(L, d_swf32, ns, d_gmax, stream);
   // non-solvent molecules first, the solvent closes phi_b (SF_MolList1stO.cpp:280-292)
   for (int pass = 0; pass < 2; pass++)
     for (int m = 0; m < nm; m++) {

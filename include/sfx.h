@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define SFX_ABI_VERSION 1
+#define SFX_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -117,7 +117,13 @@ typedef struct sfx_options {
   int precision;          /* default precision of sfx_residual/sfx_compute_phi */
   size_t memory_budget;   /* bytes for the stored propagator; 0 = unlimited.
                              A smaller budget selects sqrt(N) checkpointing. */
+  int sweep_kernel;       /* first-order 3-D simple-cubic sweeps: SFX_SWEEP_AUTO,
+                             SFX_SWEEP_PER_RANK (one launch per rank) or
+                             SFX_SWEEP_BLOCKED (up to four ranks per launch through
+                             LDS, where every side exceeds four layers) */
 } sfx_options;
+
+enum { SFX_SWEEP_AUTO = 0, SFX_SWEEP_PER_RANK = 1, SFX_SWEEP_BLOCKED = 2 };
 
 typedef struct sfx_system sfx_system;
 

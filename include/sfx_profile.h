@@ -10,11 +10,14 @@
  * report sums the event durations and ranks per kernel kind and precision and
  * gives the algorithmic bytes of an average launch: ranks per launch x sites
  * (interior) x the words of one site-rank:
- *   SFX_K_FWD   G_s = g A[G_{s-1}]                      3 words / site
- *   SFX_K_BWD   G_inv,s = g A[G_inv,s+1], phi += G G_inv 6 words / site
- *   SFX_K_TFWD  base + tangent forward step              6 words / site
- *   SFX_K_TBWD  base + tangent backward step + compose   10 words / site
- * (a word is 8 bytes for FP64 and 4 bytes for FP32).
+ *                                                   first order  second order
+ *   SFX_K_FWD   G_s = g A[G_{s-1}]                      3            13
+ *   SFX_K_BWD   G_inv,s = g A[G_inv,s+1], phi += G G_inv 6            21
+ *   SFX_K_TFWD  base + tangent forward step              6            26
+ *   SFX_K_TBWD  base + tangent backward step + compose   10           46
+ * (a word is 8 bytes for FP64 and 4 bytes for FP32; second order moves the
+ * six direction components of every propagator, one word for g and v and
+ * two for the accumulator).
  */
 #ifndef SFX_PROFILE_H
 #define SFX_PROFILE_H

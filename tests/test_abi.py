@@ -35,7 +35,7 @@ def test_binding_covers_every_declared_symbol():
 
 def test_abi_version_and_struct_sizes():
     from field_amd import _abi, sfx
-    assert sfx.lib().sfx_abi_version() == 1
+    assert sfx.lib().sfx_abi_version() == 2
     # lattice desc: 4 ints + 3*3 ints + 4 doubles
     assert ctypes.sizeof(_abi.LatticeDesc) == 4 * 4 + 9 * 4 + 4 + 4 * 8  # 4 bytes padding before doubles
     assert ctypes.sizeof(_abi.MoleculeDesc) == 40

@@ -279,3 +279,22 @@ def test_fourier_preconditioner_inverts_homogeneous_jacobian(sfx, which):
     err = np.abs(w - expect).max() / np.abs(expect).max()
     print(f"{which}: |P J v - v| / |v| = {err:.2e}")
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("name", ["3d_sc_mirrors_blocked", "3d_sc_melt_n24"])
+def test_per_rank_and_blocked_sweeps_agree(sfx, orc, name):
+    """sfx_options.sweep_kernel: the one-rank-per-launch and the blocked 3-D
+    sweeps give the oracle's residual (FP64, 1e-10) and the same FP32 residual
+    and JVP to FP32 rounding."""
+    lat, sysd = cases.case(name)
+    P = sfx.System(lat, sysd, sweep_kernel="per_rank")
+    B = sfx.System(lat, sysd, sweep_kernel="blocked")
+    x = cases.start(lat, sysd, seed=31)
+    v = np.random.default_rng(32).normal(0, 1, P.nvar)
+    fo = orc.System(lat, sysd).residual(x)
+    for S in (P, B):
+        _close(S.residual(x, precision="fp64"), fo)
+    f32p, f32b = P.residual(x, precision="fp32"), B.residual(x, precision="fp32")
+    assert np.abs(f32p - f32b).max() < 2e-5 * max(1.0, np.abs(fo).max())
+    jp, jb = P.jvp(x, v, precision="fp32"), B.jvp(x, v, precision="fp32")
+    assert np.abs(jp - jb).max() < 1e-4 * np.abs(jb).max()
