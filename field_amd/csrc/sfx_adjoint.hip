@@ -108,7 +108,8 @@ __global__ void __launch_bounds__(kTB) k_tprop(DevLattice L, const T* __restrict
   T scale = T(1), cfac = T(1);
   double lncum = 0;
   if constexpr (SCALED) {
-    float mprev = get_step_max(sc.maxv, sc.step - 1);
+    float mprev, unused;
+    block_step_max2(sc.maxv, sc.step - 1, nullptr, 0, mprev, unused);
     if (!(mprev > 0.f)) mprev = 1.f;
     lncum = sc.lnv[sc.step - 1] + log((double)mprev);
     scale = (T)(1.f / mprev);
@@ -632,26 +633,38 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
     auto slot6 = [&](int i) { return store + (size_t)i * M6; };
     auto dslot6 = [&](int i) { return dstore + (size_t)i * M6; };
     const double Sst = S.stiffness;
-    launch_init2<T>(L, true, gp(mo.seq[0]), vp(mo.seq[0]), slot6(0), dslot6(0), nullptr, nullptr, nullptr,
-                    fsc(1, 0), st);
+    // forward ranks in the W = T G form (tangent dW = T dG); the backward pass
+    // recovers G_f(s), dG_f(s) from W_f(s-1), dW_f(s-1)
+    launch_init2f<T>(L, true, gp(mo.seq[0]), vp(mo.seq[0]), slot6(0), dslot6(0), Sst, fsc(1, 0), st);
     auto step = [&](int s, int src, int dst) {
       S.prof.begin(2, P, st);
-      launch_prop2<T>(L, false, true, slot6(src), dslot6(src), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), slot6(dst),
-                      dslot6(dst), nullptr, nullptr, nullptr, Sst, fsc(s, 0), st);
+      launch_prop2f<T>(L, true, slot6(src), dslot6(src), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), slot6(dst),
+                       dslot6(dst), Sst, fsc(s, 0), st);
       S.prof.end(st);
     };
     chain_forward(plan, step);
     ChainReader<decltype(step)> rd{plan, step};
     k_set_ref2<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
+    auto bsc2 = [&](int step, int pair) {
+      RankScale r = bsc(step, pair);
+      if (scaled) r.pmaxv = maxf;
+      return r;
+    };
     int tN = mo.seq[N - 1];
-    const int iN = rd.get(N);
-    launch_init2<T>(L, true, gp(tN), vp(tN), pp6[0], dpp6[0], slot6(iN), dslot6(iN), daccp(tN), bsc(1, N), st);
+    const int iN = rd.get(N - 1);
+    launch_init2b<T>(L, true, gp(tN), vp(tN), pp6[0], dpp6[0], slot6(iN), dslot6(iN), daccp(tN), bsc2(1, N), st);
     for (int s = N - 1; s >= 1; s--) {
       int k = N - s + 1, t = mo.seq[s - 1];
-      const int is = rd.get(s);
+      const T* wp = nullptr;
+      const T* dwp = nullptr;
+      if (s >= 2) {
+        const int is = rd.get(s - 1);
+        wp = slot6(is);
+        dwp = dslot6(is);
+      }
       S.prof.begin(3, P, st);
-      launch_prop2<T>(L, true, true, pp6[(k - 2) & 1], dpp6[(k - 2) & 1], gp(t), vp(t), pp6[(k - 1) & 1],
-                      dpp6[(k - 1) & 1], slot6(is), dslot6(is), daccp(t), Sst, bsc(k, s), st);
+      launch_prop2b<T>(L, true, pp6[(k - 2) & 1], dpp6[(k - 2) & 1], gp(t), vp(t), pp6[(k - 1) & 1],
+                       dpp6[(k - 1) & 1], wp, dwp, daccp(t), Sst, bsc2(k, s), st);
       S.prof.end(st);
     }
     int nb = launch_wsum6<T>(L, pp6[(N - 1) & 1], dpp6[(N - 1) & 1], S.d_partial, W.partial2.p, st);

@@ -108,9 +108,9 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
   for (int k = 0; k < kB3Ranks; k++) cfac[k] = T(1);
   if constexpr (SCALED) {
     const int lane = t & 63;
-    float m0 = get_step_max(a.maxv, a.s0);
+    float m0, gm;
+    block_step_max2(a.maxv, a.s0, a.gmax, 0, m0, gm);
     if (!(m0 > 0.f)) m0 = 1.f;
-    float gm = get_step_max(a.gmax, 0);
     if (!(gm > 0.f)) gm = 1.f;
     in_scale = (T)(1.f / m0);
     ginv = (T)(1.f / gm);

@@ -29,6 +29,30 @@ __device__ __forceinline__ void put_step_max(unsigned* maxv, int step, float m) 
             __float_as_uint(m));
 }
 
+// the maxima of two steps (a second pointer may be null: its value is 0),
+// read once per workgroup by its first wave (one slot line per lane) and shared
+// through LDS; every thread of the workgroup must call it
+__device__ __forceinline__ void block_step_max2(const unsigned* maxa, int sa, const unsigned* maxb, int sb,
+                                                float& ma, float& mb) {
+  __shared__ float sh[2];
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    float a = __uint_as_float(maxa[(long long)sa * kStepWords + lane * kSlotStride]);
+    float b = maxb ? __uint_as_float(maxb[(long long)sb * kStepWords + lane * kSlotStride]) : 0.f;
+    for (int o = 32; o > 0; o >>= 1) {
+      a = fmaxf(a, __shfl_xor(a, o, 64));
+      b = fmaxf(b, __shfl_xor(b, o, 64));
+    }
+    if (lane == 0) {
+      sh[0] = a;
+      sh[1] = b;
+    }
+  }
+  __syncthreads();
+  ma = sh[0];
+  mb = sh[1];
+}
+
 // called by every lane of a wave; all lanes get the maximum
 __device__ __forceinline__ float get_step_max(const unsigned* maxv, int step) {
   const int lane = threadIdx.x & 63;
@@ -44,6 +68,7 @@ struct RankScale {
   const double* ref;
   int step;
   int pair;
+  const unsigned* pmaxv;  // step maxima of the partner chain (second order recovers its rank)
 };
 
 struct MolParams {

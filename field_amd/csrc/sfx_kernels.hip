@@ -94,7 +94,8 @@ __global__ void k_init_rank(DevLattice L, const T* __restrict__ g, T* __restrict
 
 // scale factor of step `step`: out values are multiplied by 1/max(previous step)
 __device__ __forceinline__ float step_scale(const RankScale& sc, double& lncum) {
-  float mprev = get_step_max(sc.maxv, sc.step - 1);
+  float mprev, unused;
+  block_step_max2(sc.maxv, sc.step - 1, nullptr, 0, mprev, unused);
   if (!(mprev > 0.f)) mprev = 1.f;  // all-zero propagator: keep it unscaled
   lncum = sc.lnv[sc.step - 1] + log((double)mprev);
   return 1.f / mprev;

@@ -71,17 +71,83 @@ __device__ __forceinline__ void transfer_rows(T S, T Lw, const T in[6], T y[6]) 
   for (int k = 0; k < 6; k++) y[k] = S * in[k] + Lw * (sum - in[k] - in[5 - k]);
 }
 
-// forward: out^k(r) = g(r) y^k(r - e_k), y = T in (y of the reversed component across a mirror)
+// the six neighbours r + e_d of a site (e = {+x, +y, +z, -z, -y, -x}) with the
+// component reversal of a mirror halo, or -1 across a surface (nb2)
+__device__ __forceinline__ void neighbours(const DevLattice& L, int x, int y, int z, long long nbr[6],
+                                           bool fl[6]) {
+#pragma unroll
+  for (int d = 0; d < 6; d++) nbr[d] = nb2(L, x, y, z, kAxis[d], kSign[d], fl[d]);
+}
+
+// T-row k of a rank at the neighbour r - e_k = r + e_{5-k}, read from the
+// rank's stored W = T G form: W^k there, W^{5-k} across a mirror, 0 across a
+// surface; a curved radial step mixes in W^{5-k} at r itself
+template <typename T, bool CURVED>
+__device__ __forceinline__ T fwd_row(const T* __restrict__ W, long long M, const long long nbr[6], const bool fl[6],
+                                     long long r, int k, T up, T dn) {
+  const long long nbi = nbr[5 - k];
+  T a = nbi < 0 ? T(0) : W[(long long)(fl[5 - k] ? 5 - k : k) * M + nbi];
+  if constexpr (CURVED) {
+    if (k == 0 || k == 5) {
+      const T f = k == 0 ? dn : up;
+      a = f * a + (T(1) - f) * W[(long long)(5 - k) * M + r];
+    }
+  }
+  return a;
+}
+
+template <typename T, bool CURVED>
+__device__ __forceinline__ void curved_fractions(const DevLattice& L, int x, T& up, T& dn) {
+  up = T(1);
+  dn = T(1);
+  if constexpr (CURVED) {
+    up = (T)(6.0 * L.lp[x]);
+    dn = (T)(6.0 * L.lm[x]);
+  }
+}
+
+// the forward partner G_f(s) at site r (and its tangent), recovered from the
+// stored W_f(s-1) exactly as the forward step produced it: g(r) times the
+// forward step's scale times the gathered row; rank 1 (Wp null) is g itself
+template <typename T, bool TANGENT, bool CURVED>
+__device__ __forceinline__ void pair_rank(const DevLattice& L, const T* __restrict__ Wp, const T* __restrict__ dWp,
+                                          T graw, T vv, long long M, int x, const long long nbr[6],
+                                          const bool fl[6], long long r, T pscale, T gp[6], T dgp[6]) {
+  if (!Wp) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      gp[k] = graw;
+      dgp[k] = -vv * graw;
+    }
+    return;
+  }
+  const T gvp = graw * pscale;
+  T up, dn;
+  curved_fractions<T, CURVED>(L, x, up, dn);
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const T a = fwd_row<T, CURVED>(Wp, M, nbr, fl, r, k, up, dn);
+    gp[k] = gvp * a;
+    if constexpr (TANGENT) {
+      const T da = fwd_row<T, CURVED>(dWp, M, nbr, fl, r, k, up, dn);
+      dgp[k] = gvp * (da - vv * a);
+    }
+  }
+}
+
+// forward: G^k_s(r) = g(r) (T G_{s-1})^k(r - e_k) from the stored W_{s-1};
+// the rank is stored as W_s = T G_s (tangent: dW_s = T dG_s)
 template <typename T, bool SCALED, bool TANGENT, bool CURVED>
-__global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restrict__ Gin, const T* __restrict__ dGin,
+__global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restrict__ Win, const T* __restrict__ dWin,
                                                 const T* __restrict__ g, const T* __restrict__ v,
-                                                T* __restrict__ Gout, T* __restrict__ dGout, T S, RankScale sc) {
+                                                T* __restrict__ Wout, T* __restrict__ dWout, T S, RankScale sc) {
   long long q = site_ordinal();
   const long long M = L.M;
   T scale = T(1);
   double lncum = 0;
   if constexpr (SCALED) {
-    float mp = get_step_max(sc.maxv, sc.step - 1);
+    float mp, unused;
+    block_step_max2(sc.maxv, sc.step - 1, nullptr, 0, mp, unused);
     if (!(mp > 0.f)) mp = 1.f;
     lncum = sc.lnv[sc.step - 1] + log((double)mp);
     scale = (T)(1.f / mp);
@@ -91,49 +157,33 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
   if (q < L.V) {
     int x, y, z;
     interior_coords(L, q, x, y, z);
-    long long r = sidx(L, x, y, z);
-    T gv = g[r] * scale;
-    T vv = TANGENT ? v[r] : T(0);
-    // curved: T-rows of the site itself (the part of a radial step that stays)
-    T ys[6], dys[6];
-    T up = T(1), dn = T(1);
-    if constexpr (CURVED) {
-      up = (T)(6.0 * L.lp[x]);
-      dn = (T)(6.0 * L.lm[x]);
-      T self[6];
-#pragma unroll
-      for (int j = 0; j < 6; j++) self[j] = Gin[(long long)j * M + r];
-      transfer_rows(S, Lw, self, ys);
-      if constexpr (TANGENT) {
-#pragma unroll
-        for (int j = 0; j < 6; j++) self[j] = dGin[(long long)j * M + r];
-        transfer_rows(S, Lw, self, dys);
-      }
-    }
+    const long long r = sidx(L, x, y, z);
+    const T gv = g[r] * scale;
+    const T vv = TANGENT ? v[r] : T(0);
+    T up, dn;
+    curved_fractions<T, CURVED>(L, x, up, dn);
+    long long nbr[6];
+    bool fl[6];
+    neighbours(L, x, y, z, nbr, fl);
+    T G[6], dG[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-      bool flip;
-      long long nbi = nb2(L, x, y, z, kAxis[k], -kSign[k], flip);
-      T in[6], yk[6];
-#pragma unroll
-      for (int j = 0; j < 6; j++) in[j] = nbi < 0 ? T(0) : Gin[(long long)j * M + nbi];
-      transfer_rows(S, Lw, in, yk);
-      T a = flip ? yk[5 - k] : yk[k];
-      const bool radial = CURVED && (k == 0 || k == 5);
-      const T f = k == 0 ? dn : up;
-      if (radial) a = f * a + (T(1) - f) * ys[5 - k];
-      T o = gv * a;
-      Gout[(long long)k * M + r] = o;
-      m = fmaxf(m, (float)o);
+      const T a = fwd_row<T, CURVED>(Win, M, nbr, fl, r, k, up, dn);
+      G[k] = gv * a;
+      m = fmaxf(m, (float)G[k]);
       if constexpr (TANGENT) {
-        T din[6], dyk[6];
-#pragma unroll
-        for (int j = 0; j < 6; j++) din[j] = nbi < 0 ? T(0) : dGin[(long long)j * M + nbi];
-        transfer_rows(S, Lw, din, dyk);
-        T da = flip ? dyk[5 - k] : dyk[k];
-        if (radial) da = f * da + (T(1) - f) * dys[5 - k];
-        dGout[(long long)k * M + r] = gv * (da - vv * a);
+        const T da = fwd_row<T, CURVED>(dWin, M, nbr, fl, r, k, up, dn);
+        dG[k] = gv * (da - vv * a);
       }
+    }
+    T w[6];
+    transfer_rows(S, Lw, G, w);
+#pragma unroll
+    for (int k = 0; k < 6; k++) Wout[(long long)k * M + r] = w[k];
+    if constexpr (TANGENT) {
+      transfer_rows(S, Lw, dG, w);
+#pragma unroll
+      for (int k = 0; k < 6; k++) dWout[(long long)k * M + r] = w[k];
     }
   }
   if constexpr (SCALED) {
@@ -150,22 +200,26 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
 }
 
 // backward: out = g T z with z_j = G^j(r + e_j) (reversed component across a
-// mirror), then acc += c sum_k out^k Gp^k (tangent: dacc)
+// mirror), then acc += c sum_k out^k G_f^k (tangent: dacc) with the forward
+// partner recovered from W_f(s-1)
 template <typename T, bool SCALED, bool TANGENT, bool CURVED>
 __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restrict__ Gin, const T* __restrict__ dGin,
                                                 const T* __restrict__ g, const T* __restrict__ v,
                                                 T* __restrict__ Gout, T* __restrict__ dGout,
-                                                const T* __restrict__ Gp, const T* __restrict__ dGp,
+                                                const T* __restrict__ Wp, const T* __restrict__ dWp,
                                                 T* __restrict__ acc, T S, RankScale sc) {
   long long q = site_ordinal();
   const long long M = L.M;
-  T scale = T(1), cfac = T(1);
+  T scale = T(1), cfac = T(1), pscale = T(1);
   double lncum = 0;
   if constexpr (SCALED) {
-    float mp = get_step_max(sc.maxv, sc.step - 1);
+    float mp, mpp;
+    block_step_max2(sc.maxv, sc.step - 1, Wp ? sc.pmaxv : nullptr, sc.pair - 1, mp, mpp);
     if (!(mp > 0.f)) mp = 1.f;
+    if (!(mpp > 0.f)) mpp = 1.f;
     lncum = sc.lnv[sc.step - 1] + log((double)mp);
     scale = (T)(1.f / mp);
+    pscale = (T)(1.f / mpp);
     cfac = (T)exp(lncum + sc.lnpair[sc.pair] - *sc.ref);
   }
   const T Lw = (T(1) - S) / T(4);
@@ -174,12 +228,14 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
     int x, y, z;
     interior_coords(L, q, x, y, z);
     long long r = sidx(L, x, y, z);
+    long long nbr[6];
+    bool fl[6];
+    neighbours(L, x, y, z, nbr, fl);
     T zz[6], dz[6];
 #pragma unroll
     for (int j = 0; j < 6; j++) {
-      bool flip;
-      long long nbi = nb2(L, x, y, z, kAxis[j], kSign[j], flip);
-      int jj = flip ? 5 - j : j;
+      const long long nbi = nbr[j];
+      const int jj = fl[j] ? 5 - j : j;
       zz[j] = nbi < 0 ? T(0) : Gin[(long long)jj * M + nbi];
       if constexpr (TANGENT) dz[j] = nbi < 0 ? T(0) : dGin[(long long)jj * M + nbi];
     }
@@ -198,20 +254,22 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
     T o[6], d[6];
     transfer_rows(S, Lw, zz, o);
     if constexpr (TANGENT) transfer_rows(S, Lw, dz, d);
-    T gv = g[r] * scale;
-    T vv = TANGENT ? v[r] : T(0);
+    const T graw = g[r];
+    const T gv = graw * scale;
+    const T vv = TANGENT ? v[r] : T(0);
+    T gp[6], dgp[6];
+    pair_rank<T, TANGENT, CURVED>(L, Wp, dWp, graw, vv, M, x, nbr, fl, r, pscale, gp, dgp);
     T a = T(0), da = T(0);
 #pragma unroll
     for (int k = 0; k < 6; k++) {
       T ok = gv * o[k];
       Gout[(long long)k * M + r] = ok;
       m = fmaxf(m, (float)ok);
-      T gp = Gp[(long long)k * M + r];
-      a += ok * gp;
+      a += ok * gp[k];
       if constexpr (TANGENT) {
         T dk = gv * (d[k] - vv * o[k]);
         dGout[(long long)k * M + r] = dk;
-        da += dk * gp + ok * dGp[(long long)k * M + r];
+        da += dk * gp[k] + ok * dgp[k];
       }
     }
     if constexpr (TANGENT)
@@ -232,40 +290,92 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
   }
 }
 
-// rank 1 / rank N: all six components equal g (SF_Homopol2ndO.cpp:167, 176);
-// tangent -v g; optional composition with the stored rank
+// rank 1 of the forward chain: all six components of G are g
+// (SF_Homopol2ndO.cpp:167; tangent -v g), stored in the W = T G form
 template <typename T, bool TANGENT>
-__global__ void k_init2(DevLattice L, const T* __restrict__ g, const T* __restrict__ v, T* __restrict__ G,
-                        T* __restrict__ dG, const T* __restrict__ Gp, const T* __restrict__ dGp,
-                        T* __restrict__ acc, RankScale sc) {
+__global__ void k_init2f(DevLattice L, const T* __restrict__ g, const T* __restrict__ v, T* __restrict__ W,
+                         T* __restrict__ dW, T S, RankScale sc) {
   long long q = site_ordinal();
   const long long M = L.M;
+  const T Lw = (T(1) - S) / T(4);
   float m = 0.f;
   if (q < L.V) {
     int x, y, z;
     interior_coords(L, q, x, y, z);
     long long r = sidx(L, x, y, z);
-    T gv = g[r];
-    T vv = TANGENT ? v[r] : T(0);
-    T a = T(0);
-    for (int k = 0; k < 6; k++) {
-      G[(long long)k * M + r] = gv;
-      if constexpr (TANGENT) dG[(long long)k * M + r] = -vv * gv;
-      if (Gp) {
-        if constexpr (TANGENT)
-          a += -vv * gv * Gp[(long long)k * M + r] + gv * dGp[(long long)k * M + r];
-        else
-          a += gv * Gp[(long long)k * M + r];
-      }
-    }
-    if (acc) {
-      T c = T(1);
-      if (sc.maxv) c = (T)exp(sc.lnpair[sc.pair] - *sc.ref);
-      acc[r] += a * c;
+    const T gv = g[r];
+    T G[6], w[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) G[k] = gv;
+    transfer_rows(S, Lw, G, w);
+#pragma unroll
+    for (int k = 0; k < 6; k++) W[(long long)k * M + r] = w[k];
+    if constexpr (TANGENT) {
+      const T vv = v[r];
+#pragma unroll
+      for (int k = 0; k < 6; k++) G[k] = -vv * gv;
+      transfer_rows(S, Lw, G, w);
+#pragma unroll
+      for (int k = 0; k < 6; k++) dW[(long long)k * M + r] = w[k];
     }
     m = (float)gv;
   }
   if (sc.maxv) {
+    __shared__ float sm[kB2 / 64];
+    m = wmax2(m);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = fmaxf(m, sm[i]);
+      put_step_max(sc.maxv, sc.step, m);
+      if (blockIdx.x == 0) sc.lnv[sc.step] = 0.0;
+    }
+  }
+}
+
+// rank N of the backward chain: all six components are g (SF_Homopol2ndO.cpp:176;
+// tangent -v g), composed with the forward partner recovered from Wp = W_f(N-1)
+template <typename T, bool SCALED, bool TANGENT, bool CURVED>
+__global__ void k_init2b(DevLattice L, const T* __restrict__ g, const T* __restrict__ v, T* __restrict__ G,
+                         T* __restrict__ dG, const T* __restrict__ Wp, const T* __restrict__ dWp,
+                         T* __restrict__ acc, RankScale sc) {
+  long long q = site_ordinal();
+  const long long M = L.M;
+  T pscale = T(1);
+  if constexpr (SCALED) {
+    float mpp, unused;
+    block_step_max2(sc.pmaxv, sc.pair - 1, nullptr, 0, mpp, unused);
+    if (!(mpp > 0.f)) mpp = 1.f;
+    pscale = (T)(1.f / mpp);
+  }
+  float m = 0.f;
+  if (q < L.V) {
+    int x, y, z;
+    interior_coords(L, q, x, y, z);
+    long long r = sidx(L, x, y, z);
+    const T gv = g[r];
+    const T vv = TANGENT ? v[r] : T(0);
+    long long nbr[6];
+    bool fl[6];
+    neighbours(L, x, y, z, nbr, fl);
+    T gp[6], dgp[6];
+    pair_rank<T, TANGENT, CURVED>(L, Wp, dWp, gv, vv, M, x, nbr, fl, r, pscale, gp, dgp);
+    T a = T(0);
+    for (int k = 0; k < 6; k++) {
+      G[(long long)k * M + r] = gv;
+      if constexpr (TANGENT) {
+        dG[(long long)k * M + r] = -vv * gv;
+        a += -vv * gv * gp[k] + gv * dgp[k];
+      } else {
+        a += gv * gp[k];
+      }
+    }
+    T c = T(1);
+    if constexpr (SCALED) c = (T)exp(sc.lnpair[sc.pair] - *sc.ref);
+    acc[r] += a * c;
+    m = (float)gv;
+  }
+  if constexpr (SCALED) {
     __shared__ float sm[kB2 / 64];
     m = wmax2(m);
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
@@ -320,54 +430,70 @@ __global__ void k_wsum6(DevLattice L, const T* __restrict__ G, const T* __restri
 }
 
 // ---------------------------------------------------------------------------
+#define SFX_O2_DISPATCH(K, ...)                 \
+  do {                                          \
+    if (sc_on) {                                \
+      if (tangent) {                            \
+        if (L.curved) K(true, true, true);      \
+        else K(true, true, false);              \
+      } else {                                  \
+        if (L.curved) K(true, false, true);     \
+        else K(true, false, false);             \
+      }                                         \
+    } else {                                    \
+      if (tangent) {                            \
+        if (L.curved) K(false, true, true);     \
+        else K(false, true, false);             \
+      } else {                                  \
+        if (L.curved) K(false, false, true);    \
+        else K(false, false, false);            \
+      }                                         \
+    }                                           \
+  } while (0)
+
 template <typename T>
-void launch_prop2(const DevLattice& L, bool backward, bool tangent, const T* Gin, const T* dGin, const T* g,
-                  const T* v, T* Gout, T* dGout, const T* Gp, const T* dGp, T* acc, double S, const RankScale& sc,
-                  hipStream_t st) {
-  unsigned nb = grid1(L.V, kB2);
-  bool sc_on = sc.maxv != nullptr;
-  T s = (T)S;
-#define F2(SC, TG, CV) k_prop2f<T, SC, TG, CV><<<nb, kB2, 0, st>>>(L, Gin, dGin, g, v, Gout, dGout, s, sc)
-#define B2(SC, TG, CV) \
-  k_prop2b<T, SC, TG, CV><<<nb, kB2, 0, st>>>(L, Gin, dGin, g, v, Gout, dGout, Gp, dGp, acc, s, sc)
-#define F(SC, TG) \
-  do {                                   \
-    if (L.curved) F2(SC, TG, true);      \
-    else F2(SC, TG, false);              \
-  } while (0)
-#define B(SC, TG) \
-  do {                                   \
-    if (L.curved) B2(SC, TG, true);      \
-    else B2(SC, TG, false);              \
-  } while (0)
-  if (!backward) {
-    if (sc_on) {
-      if (tangent) F(true, true); else F(true, false);
-    } else {
-      if (tangent) F(false, true); else F(false, false);
-    }
-  } else {
-    if (sc_on) {
-      if (tangent) B(true, true); else B(true, false);
-    } else {
-      if (tangent) B(false, true); else B(false, false);
-    }
-  }
-#undef F
-#undef B
-#undef F2
-#undef B2
+void launch_prop2f(const DevLattice& L, bool tangent, const T* Win, const T* dWin, const T* g, const T* v, T* Wout,
+                   T* dWout, double S, const RankScale& sc, hipStream_t st) {
+  const unsigned nb = grid1(L.V, kB2);
+  const bool sc_on = sc.maxv != nullptr;
+  const T s = (T)S;
+#define K(SC, TG, CV) k_prop2f<T, SC, TG, CV><<<nb, kB2, 0, st>>>(L, Win, dWin, g, v, Wout, dWout, s, sc)
+  SFX_O2_DISPATCH(K);
+#undef K
 }
 
 template <typename T>
-void launch_init2(const DevLattice& L, bool tangent, const T* g, const T* v, T* G, T* dG, const T* Gp,
-                  const T* dGp, T* acc, const RankScale& sc, hipStream_t st) {
-  unsigned nb = grid1(L.V, kB2);
-  if (tangent)
-    k_init2<T, true><<<nb, kB2, 0, st>>>(L, g, v, G, dG, Gp, dGp, acc, sc);
-  else
-    k_init2<T, false><<<nb, kB2, 0, st>>>(L, g, v, G, dG, Gp, dGp, acc, sc);
+void launch_prop2b(const DevLattice& L, bool tangent, const T* Gin, const T* dGin, const T* g, const T* v, T* Gout,
+                   T* dGout, const T* Wp, const T* dWp, T* acc, double S, const RankScale& sc, hipStream_t st) {
+  const unsigned nb = grid1(L.V, kB2);
+  const bool sc_on = sc.maxv != nullptr;
+  const T s = (T)S;
+#define K(SC, TG, CV) \
+  k_prop2b<T, SC, TG, CV><<<nb, kB2, 0, st>>>(L, Gin, dGin, g, v, Gout, dGout, Wp, dWp, acc, s, sc)
+  SFX_O2_DISPATCH(K);
+#undef K
 }
+
+template <typename T>
+void launch_init2f(const DevLattice& L, bool tangent, const T* g, const T* v, T* W, T* dW, double S,
+                   const RankScale& sc, hipStream_t st) {
+  const unsigned nb = grid1(L.V, kB2);
+  if (tangent)
+    k_init2f<T, true><<<nb, kB2, 0, st>>>(L, g, v, W, dW, (T)S, sc);
+  else
+    k_init2f<T, false><<<nb, kB2, 0, st>>>(L, g, v, W, dW, (T)S, sc);
+}
+
+template <typename T>
+void launch_init2b(const DevLattice& L, bool tangent, const T* g, const T* v, T* G, T* dG, const T* Wp,
+                   const T* dWp, T* acc, const RankScale& sc, hipStream_t st) {
+  const unsigned nb = grid1(L.V, kB2);
+  const bool sc_on = sc.maxv != nullptr;
+#define K(SC, TG, CV) k_init2b<T, SC, TG, CV><<<nb, kB2, 0, st>>>(L, g, v, G, dG, Wp, dWp, acc, sc)
+  SFX_O2_DISPATCH(K);
+#undef K
+}
+#undef SFX_O2_DISPATCH
 
 template <typename T>
 int launch_wsum6(const DevLattice& L, const T* G, const T* dG, double* p1, double* p2, hipStream_t st) {
@@ -376,16 +502,24 @@ int launch_wsum6(const DevLattice& L, const T* G, const T* dG, double* p1, doubl
   return (int)nb;
 }
 
-template void launch_prop2<double>(const DevLattice&, bool, bool, const double*, const double*, const double*,
-                                   const double*, double*, double*, const double*, const double*, double*, double,
+template void launch_prop2f<double>(const DevLattice&, bool, const double*, const double*, const double*,
+                                    const double*, double*, double*, double, const RankScale&, hipStream_t);
+template void launch_prop2f<float>(const DevLattice&, bool, const float*, const float*, const float*, const float*,
+                                   float*, float*, double, const RankScale&, hipStream_t);
+template void launch_prop2b<double>(const DevLattice&, bool, const double*, const double*, const double*,
+                                    const double*, double*, double*, const double*, const double*, double*, double,
+                                    const RankScale&, hipStream_t);
+template void launch_prop2b<float>(const DevLattice&, bool, const float*, const float*, const float*, const float*,
+                                   float*, float*, const float*, const float*, float*, double, const RankScale&,
+                                   hipStream_t);
+template void launch_init2f<double>(const DevLattice&, bool, const double*, const double*, double*, double*, double,
+                                    const RankScale&, hipStream_t);
+template void launch_init2f<float>(const DevLattice&, bool, const float*, const float*, float*, float*, double,
                                    const RankScale&, hipStream_t);
-template void launch_prop2<float>(const DevLattice&, bool, bool, const float*, const float*, const float*,
-                                  const float*, float*, float*, const float*, const float*, float*, double,
-                                  const RankScale&, hipStream_t);
-template void launch_init2<double>(const DevLattice&, bool, const double*, const double*, double*, double*,
-                                   const double*, const double*, double*, const RankScale&, hipStream_t);
-template void launch_init2<float>(const DevLattice&, bool, const float*, const float*, float*, float*, const float*,
-                                  const float*, float*, const RankScale&, hipStream_t);
+template void launch_init2b<double>(const DevLattice&, bool, const double*, const double*, double*, double*,
+                                    const double*, const double*, double*, const RankScale&, hipStream_t);
+template void launch_init2b<float>(const DevLattice&, bool, const float*, const float*, float*, float*, const float*,
+                                   const float*, float*, const RankScale&, hipStream_t);
 template int launch_wsum6<double>(const DevLattice&, const double*, const double*, double*, double*, hipStream_t);
 template int launch_wsum6<float>(const DevLattice&, const float*, const float*, double*, double*, hipStream_t);
 

@@ -417,24 +417,30 @@ void System::sweep(int m, bool scaled) {
     const long long M6 = 6 * Mn;
     T* pp6[2] = {(T*)d_pp, (T*)d_pp + M6};
     auto slot6 = [&](int i) { return store + (size_t)i * M6; };
-    launch_init2<T>(L, false, gp(mo.seq[0]), nullptr, slot6(0), nullptr, nullptr, nullptr, nullptr, fsc(1, 0), st);
+    // forward ranks in the W = T G form; the backward pass recovers G_f(s) from W_f(s-1)
+    launch_init2f<T>(L, false, gp(mo.seq[0]), nullptr, slot6(0), nullptr, stiffness, fsc(1, 0), st);
     auto step = [&](int s, int src, int dst) {
       prof.begin(0, P, st);
-      launch_prop2<T>(L, false, false, slot6(src), nullptr, gp(mo.seq[s - 1]), nullptr, slot6(dst), nullptr,
-                      nullptr, nullptr, nullptr, stiffness, fsc(s, 0), st);
+      launch_prop2f<T>(L, false, slot6(src), nullptr, gp(mo.seq[s - 1]), nullptr, slot6(dst), nullptr, stiffness,
+                       fsc(s, 0), st);
       prof.end(st);
     };
     chain_forward(plan, step);
     ChainReader<decltype(step)> rd{plan, step};
     k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
-    launch_init2<T>(L, false, gp(mo.seq[N - 1]), nullptr, pp6[0], nullptr, slot6(rd.get(N)), nullptr,
-                    accp(mo.seq[N - 1]), bsc(1, N), st);
+    auto bsc2 = [&](int step, int pair) {
+      RankScale r = bsc(step, pair);
+      if (scaled) r.pmaxv = maxf;
+      return r;
+    };
+    launch_init2b<T>(L, false, gp(mo.seq[N - 1]), nullptr, pp6[0], nullptr, slot6(rd.get(N - 1)), nullptr,
+                     accp(mo.seq[N - 1]), bsc2(1, N), st);
     for (int s = N - 1; s >= 1; s--) {
       int k = N - s + 1;
-      const T* pair = slot6(rd.get(s));
+      const T* wp = s >= 2 ? slot6(rd.get(s - 1)) : nullptr;
       prof.begin(1, P, st);
-      launch_prop2<T>(L, true, false, pp6[(k - 2) & 1], nullptr, gp(mo.seq[s - 1]), nullptr, pp6[(k - 1) & 1],
-                      nullptr, pair, nullptr, accp(mo.seq[s - 1]), stiffness, bsc(k, s), st);
+      launch_prop2b<T>(L, false, pp6[(k - 2) & 1], nullptr, gp(mo.seq[s - 1]), nullptr, pp6[(k - 1) & 1], nullptr,
+                       wp, nullptr, accp(mo.seq[s - 1]), stiffness, bsc2(k, s), st);
       prof.end(st);
     }
     int nb = launch_wsum6<T>(L, pp6[(N - 1) & 1], nullptr, d_partial, nullptr, st);
