@@ -86,6 +86,7 @@ class SolverOpts(C.Structure):
         ("precondition", C.c_int),
         ("ptc_dt0", C.c_double),
         ("verbose", C.c_int),
+        ("diis_backtrack", C.c_int),
     ]
 
 

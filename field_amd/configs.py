@@ -121,6 +121,69 @@ def batch_1d(count=512, seed0=4):
     return out
 
 
+def _fh_bisect(fn, lo, hi, iters=200):
+    """Root of fn on [lo, hi] with fn(lo) < 0 < fn(hi), by bisection."""
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        if fn(mid) < 0:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def fh_binodal(N, chi):
+    """Coexisting volume fractions (dilute, concentrated) of A_N in a monomeric
+    solvent under Flory-Huggins, f(phi) = phi/N ln phi + (1-phi) ln(1-phi)
+    + chi phi (1-phi) per site, or None when chi <= chi_c = (1 + N^-1/2)^2 / 2.
+
+    The two phases share f'(phi) (exchange potential) and f - phi f'
+    (osmotic pressure). The dilute root is found in ln phi, since it falls far
+    below 1e-300 for long chains in a poor solvent."""
+    if chi <= 0.5 * (1.0 + N ** -0.5) ** 2:
+        return None
+
+    def d1(la):  # f'(phi) at phi = exp(la)
+        p = math.exp(la)
+        return (la + 1.0) / N - math.log1p(-p) - 1.0 + chi * (1.0 - 2.0 * p)
+
+    def press(la):  # f - phi f' at phi = exp(la)
+        p = math.exp(la)
+        return math.log1p(-p) + p * (1.0 - 1.0 / N) + chi * p * p
+
+    pc = 1.0 / (1.0 + math.sqrt(N))
+    spin = lambda p: 1.0 / (N * p) + 1.0 / (1.0 - p) - 2.0 * chi  # f'' = 0
+    s1 = _fh_bisect(lambda p: -spin(p), 1e-300, pc)
+    s2 = _fh_bisect(spin, pc, 1.0 - 1e-16)
+    ls1, ls2 = math.log(s1), math.log(s2)
+
+    # f' rises on (0, s1) and on (s2, 1) and falls in between, so every common
+    # slope t in (f'(s2), f'(s1)) has one root a(t) < s1 and one b(t) > s2. The
+    # tangent intercepts differ by h(t) = press(a) - press(b) with
+    # dh/dt = b - a > 0: h has one root, the binodal.
+    def roots(t):
+        la = _fh_bisect(lambda v: d1(v) - t, -1e6, ls1)
+        lb = _fh_bisect(lambda v: d1(v) - t, ls2, math.log1p(-1e-16))
+        return la, lb
+
+    def h(t):
+        la, lb = roots(t)
+        return press(la) - press(lb)
+
+    la, lb = roots(_fh_bisect(h, d1(ls2), d1(ls1)))
+    return math.exp(la), math.exp(lb)
+
+
+def bulk_is_stable(p):
+    """True when the bulk of a batch_1d-style problem is a stable one-phase
+    Flory-Huggins solution: chi_as <= chi_c, or phi_b below the dilute binodal.
+    Otherwise the bulk is metastable or unstable, the wall nucleates a
+    condensed layer that grows with the box, and a converged profile need not
+    exist."""
+    b = fh_binodal(p["N"], p["chi_as"])
+    return b is None or p["phibulk"] <= b[0]
+
+
 def adsorption_problem(p):
     """(lattice, system, zero start) of one batch_1d-style problem dict."""
     return adsorption_1d(p["n_layers"], p["N"], phibulk=p["phibulk"], chi_as=p["chi_as"],

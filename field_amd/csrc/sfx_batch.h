@@ -21,8 +21,9 @@ struct BatchProb {
 
 // DIIS bookkeeping per problem (MPS.cpp:263-282 variables).
 struct BatchState {
-  int it, k, k_diis, done;  // done: 0 running, 1 converged, 2 iteration limit, 3 non-finite
+  int it, k, k_diis, done;  // done: 0 running, 1 converged, 2 iteration limit, 3 non-finite / given up
   double err;
+  double eta, best;         // this problem's step and best residual norm (backtracking)
 };
 
 struct Batch {
@@ -49,6 +50,7 @@ struct Batch {
   int graph_key[3] = {0, 0, 0};  // block, diis_m, iterations per replay of the captured graph
   double graph_tol = -1, graph_eta = 0;
   int graph_max_iter = -1;
+  int graph_backtrack = -1;
   // results of the last solve
   std::vector<BatchState> last;
   int residual_launches = 0;
@@ -62,7 +64,7 @@ struct Batch {
  private:
   void ensure_history(int m);
   void launch_residual(int bt, const int* active);
-  void launch_diis(int bt, int m, double eta, double tol, int max_iter);
+  void launch_diis(int bt, int m, double eta, double tol, int max_iter, int backtrack);
   void drop_graph();
 };
 

@@ -232,17 +232,23 @@ __device__ void diis_solve(double* A, double* rhs, int k, double* X) {
 
 // One DIIS step per active problem given g = f(x) at its current x
 // (MPS.cpp:263-282; the start plays x0, so the first step is x_x0[0]).
-// History per problem: x0 | xR[m] | x_x0[m] | A[m*m].
+// History per problem: x0 | xR[m] | x_x0[m] | A[m*m] | best x. With
+// backtracking (diis_backtrack) a non-finite residual, or one kGrow times the
+// best so far, sends the problem back to its best point with its history
+// cleared and its step halved; below kEtaFloor times the initial step it
+// stops there.
+constexpr double kGrow = 100.0;
+constexpr double kEtaFloor = 1e-4;
 template <int BT>
 __global__ void __launch_bounds__(BT) k_batch_diis(const BatchProb* __restrict__ P, double* __restrict__ x,
                                                    const double* __restrict__ g, double* __restrict__ hist,
                                                    BatchState* __restrict__ S, int m, double eta, double tol,
-                                                   int max_iter, int* __restrict__ active) {
+                                                   int max_iter, int backtrack, int* __restrict__ active) {
   const int b = blockIdx.x;
   if (!active[b]) return;
   __shared__ double red[(kMaxDiis + 1) * (BT / 64)];
   __shared__ double Ap[kMaxDiis * kMaxDiis], rhs[kMaxDiis], coef[kMaxDiis];
-  __shared__ int verdict;
+  __shared__ int verdict, save_best;
   const BatchProb p = P[b];
   const long long nv = 2LL * p.n;
   double* xb = x + p.xoff;
@@ -251,12 +257,14 @@ __global__ void __launch_bounds__(BT) k_batch_diis(const BatchProb* __restrict__
   double* xR = x0 + nv;
   double* xx0 = xR + (long long)m * nv;
   double* Aij = xx0 + (long long)m * nv;
+  double* xbest = Aij + (long long)m * m;
   const BatchState st = S[b];
   const bool first = st.k_diis == 0;
-  const int it = first ? 0 : st.it + 1;
+  const int it = st.it + 1;  // evaluations so far (the reset state holds -1)
   const int k = it % m;
   const int kd = first ? 1 : std::min(st.k_diis + 1, m);
   const int t = threadIdx.x;
+  const double eta_p = backtrack ? st.eta : eta;
 
   // new point x - eta g, its step, and the dot products of the step with the kept steps
   double part[kMaxDiis + 1];
@@ -264,7 +272,7 @@ __global__ void __launch_bounds__(BT) k_batch_diis(const BatchProb* __restrict__
   for (int i = 0; i <= kMaxDiis; i++) part[i] = 0.0;
   for (long long q = t; q < nv; q += BT) {
     const double xi = xb[q], gi = gb[q];
-    const double xn = xi - eta * gi;
+    const double xn = xi - eta_p * gi;
     const double d = xn - xi;
     x0[q] = xi;
     xR[(long long)k * nv + q] = xn;
@@ -301,8 +309,16 @@ __global__ void __launch_bounds__(BT) k_batch_diis(const BatchProb* __restrict__
     const double err = sqrt(dots[kMaxDiis]);
     BatchState ns = st;
     ns.err = err;
-    int v = 0;  // 0: combine, 1: stop at the evaluated point, 2: take the plain step
-    if (!isfinite(err)) {
+    // 0: combine, 1: stop at the evaluated point, 2: take the plain step,
+    // 3: return to the best point (history cleared)
+    int v = 0, save = 0;
+    if (backtrack && isfinite(st.best) && (!isfinite(err) || err > kGrow * st.best)) {
+      v = 3;
+      ns.err = st.best;
+      ns.k_diis = 0;
+      ns.eta = 0.5 * st.eta;
+      if (ns.eta < kEtaFloor * eta) ns.done = 3;
+    } else if (!isfinite(err)) {
       ns.done = 3;
       v = 1;
     } else if (err <= tol) {
@@ -311,14 +327,20 @@ __global__ void __launch_bounds__(BT) k_batch_diis(const BatchProb* __restrict__
     } else if (first) {
       v = 2;
     }
+    if (backtrack && v != 3 && err < st.best) {
+      ns.best = err;
+      save = 1;
+    }
     ns.it = it;
-    if (first) {
+    if (v == 3) {
+      if (it >= max_iter) ns.done = 2;
+    } else if (first) {
       ns.k = 0;
       ns.k_diis = 1;
       // A[0][0] is never filled in: the first step's self product is not
       // computed in MPS.cpp:263-270 either
       for (int i = 0; i < m * m; i++) Aij[i] = 0.0;
-      if (v == 2 && max_iter == 0) ns.done = 2;
+      if (v == 2 && it >= max_iter) ns.done = 2;
     } else if (v == 0) {
       if (st.k_diis + 1 > m)  // drop the oldest row and column
         for (int i = 1; i < m; i++)
@@ -337,11 +359,18 @@ __global__ void __launch_bounds__(BT) k_batch_diis(const BatchProb* __restrict__
     if (ns.done) active[b] = 0;
     S[b] = ns;
     verdict = v;
+    save_best = save;
   }
   __syncthreads();
+  if (save_best)
+    for (long long q = t; q < nv; q += BT) xbest[q] = x0[q];  // x0 holds the evaluated point
   if (verdict == 1) return;  // x stays the point whose residual was measured
+  if (verdict == 3) {
+    for (long long q = t; q < nv; q += BT) xb[q] = xbest[q];
+    return;
+  }
   if (verdict == 2) {
-    for (long long q = t; q < nv; q += BT) xb[q] = xR[q];
+    for (long long q = t; q < nv; q += BT) xb[q] = xR[(long long)k * nv + q];
     return;
   }
   for (long long q = t; q < nv; q += BT) {
@@ -365,11 +394,11 @@ __global__ void k_batch_count(const int* __restrict__ active, int n, int* __rest
   if (threadIdx.x == 0) *count = red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ void k_batch_reset(int* __restrict__ active, BatchState* __restrict__ S, int n) {
+__global__ void k_batch_reset(int* __restrict__ active, BatchState* __restrict__ S, int n, double eta) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     active[i] = 1;
-    S[i] = BatchState{0, 0, 0, 0, 0.0};
+    S[i] = BatchState{-1, 0, 0, 0, 0.0, eta, INFINITY};
   }
 }
 
@@ -456,16 +485,19 @@ void Batch::launch_residual(int bt, const int* active) {
   SFX_HIP(hipGetLastError());
 }
 
-void Batch::launch_diis(int bt, int m, double eta, double tol, int max_iter) {
+void Batch::launch_diis(int bt, int m, double eta, double tol, int max_iter, int backtrack) {
   switch (bt) {
     case 64:
-      k_batch_diis<64><<<B, 64, 0, stream>>>(d_prob, d_x, d_f, d_hist, d_state, m, eta, tol, max_iter, d_active);
+      k_batch_diis<64><<<B, 64, 0, stream>>>(d_prob, d_x, d_f, d_hist, d_state, m, eta, tol, max_iter, backtrack,
+                                             d_active);
       break;
     case 128:
-      k_batch_diis<128><<<B, 128, 0, stream>>>(d_prob, d_x, d_f, d_hist, d_state, m, eta, tol, max_iter, d_active);
+      k_batch_diis<128><<<B, 128, 0, stream>>>(d_prob, d_x, d_f, d_hist, d_state, m, eta, tol, max_iter, backtrack,
+                                               d_active);
       break;
     default:
-      k_batch_diis<256><<<B, 256, 0, stream>>>(d_prob, d_x, d_f, d_hist, d_state, m, eta, tol, max_iter, d_active);
+      k_batch_diis<256><<<B, 256, 0, stream>>>(d_prob, d_x, d_f, d_hist, d_state, m, eta, tol, max_iter, backtrack,
+                                               d_active);
       break;
   }
   SFX_HIP(hipGetLastError());
@@ -505,7 +537,7 @@ void Batch::ensure_history(int m) {
   long long ho = 0;
   for (int i = 0; i < B; i++) {
     probs[i].hoff = ho;
-    ho += 2LL * probs[i].n * (1 + 2LL * m) + (long long)m * m;
+    ho += 2LL * probs[i].n * (2 + 2LL * m) + (long long)m * m;  // x0, xR[m], x_x0[m], A, best x
   }
   SFX_HIP(hipMalloc(&d_hist, ho * sizeof(double)));
   SFX_HIP(hipMemcpy(d_prob, probs.data(), B * sizeof(BatchProb), hipMemcpyHostToDevice));
@@ -530,19 +562,20 @@ void Batch::solve(const sfx_solver_opts& o, double* x, int where, int* iters, do
   const bool host = where == SFX_HOST;
   SFX_HIP(hipMemcpyAsync(d_x, x, nvar * sizeof(double), host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
                          stream));
-  k_batch_reset<<<grid1(B, 256), 256, 0, stream>>>(d_active, d_state, B);
+  k_batch_reset<<<grid1(B, 256), 256, 0, stream>>>(d_active, d_state, B, o.diis_eta);
   SFX_HIP(hipGetLastError());
 
   // the captured graph: R iterations (residual + DIIS step) and the count of active problems
   const int R = 8;
   const bool same = graph_exec && graph_key[0] == block && graph_key[1] == o.diis_m && graph_key[2] == R &&
-                    graph_tol == o.tolerance && graph_eta == o.diis_eta && graph_max_iter == o.max_iter;
+                    graph_tol == o.tolerance && graph_eta == o.diis_eta && graph_max_iter == o.max_iter &&
+                    graph_backtrack == o.diis_backtrack;
   if (!same) {
     drop_graph();
     SFX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     for (int r = 0; r < R; r++) {
       launch_residual(block, d_active);
-      launch_diis(block, o.diis_m, o.diis_eta, o.tolerance, o.max_iter);
+      launch_diis(block, o.diis_m, o.diis_eta, o.tolerance, o.max_iter, o.diis_backtrack);
     }
     k_batch_count<<<1, 256, 0, stream>>>(d_active, B, d_count);
     SFX_HIP(hipGetLastError());
@@ -554,6 +587,7 @@ void Batch::solve(const sfx_solver_opts& o, double* x, int where, int* iters, do
     graph_tol = o.tolerance;
     graph_eta = o.diis_eta;
     graph_max_iter = o.max_iter;
+    graph_backtrack = o.diis_backtrack;
   }
   hipEvent_t e0, e1;
   SFX_HIP(hipEventCreate(&e0));

@@ -190,6 +190,7 @@ int sfx_solver_defaults(sfx_solver_opts* o) {
   o->precondition = 1;
   o->ptc_dt0 = 1.0;
   o->verbose = 0;
+  o->diis_backtrack = 0;
   return SFX_OK;
 }
 

@@ -189,6 +189,11 @@ typedef struct sfx_solver_opts {
                              (I/dt + J) dx = -f from dt = ptc_dt0 with dt grown as ||f|| falls;
                              0 = Newton steps with a backtracking line search */
   int verbose;
+  int diis_backtrack;     /* batched DIIS: 1 = per problem, a non-finite residual or one 100x
+                             above the best so far returns to the best point, clears the DIIS
+                             history and halves that problem's step (giving up below 1e-4 of
+                             diis_eta, at the best point); 0 = the plain iteration of
+                             MPS.cpp:228-284. No counterpart in the reference. */
 } sfx_solver_opts;
 
 typedef struct sfx_solve_stats {

@@ -127,3 +127,22 @@ def test_uneven_sweep_sample(sfx, orc):
         lat, sysd, _ = configs.adsorption_problem(probs[i])
         O = orc.System(lat, sysd)
         assert np.linalg.norm(O.residual(B.split(x)[i])) <= 1e-7 * (1 + 1e-6)
+
+
+def test_uneven_sweep_backtracking(sfx, orc):
+    """The same 64 problems with per-problem backtracking: no problem ends on a
+    non-finite residual, every problem whose bulk is a stable Flory-Huggins
+    solution converges, and every converged one passes the oracle."""
+    probs = configs.batch_1d(64)
+    B = sfx.Batch(probs)
+    x, its, norms, st = B.solve(tolerance=1e-7, max_iter=3000, diis_backtrack=1, diis_eta=1.0)
+    conv = norms <= 1e-7
+    stable = np.array([configs.bulk_is_stable(p) for p in probs])
+    print(f"uneven sweep, backtracking: {conv.sum()}/{len(probs)} converged, "
+          f"{conv[stable].sum()}/{stable.sum()} with a stable bulk, {st['seconds']:.3f} s")
+    assert np.isfinite(norms).all()
+    assert conv[stable].all()
+    for i in np.flatnonzero(conv)[:16]:
+        lat, sysd, _ = configs.adsorption_problem(probs[i])
+        O = orc.System(lat, sysd)
+        assert np.linalg.norm(O.residual(B.split(x)[i])) <= 1e-7 * (1 + 1e-6)
