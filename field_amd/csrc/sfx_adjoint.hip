@@ -25,11 +25,22 @@ namespace sfx {
 
 constexpr int kTB = 256;
 
+template <typename T>
+static __global__ void k_zero(T* p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = T(0);
+}
+template <typename T>
+static void zero(T* p, long long n, hipStream_t st) {
+  k_zero<T><<<grid1(n, kTB), kTB, 0, st>>>(p, n);
+}
+
 struct Tangent {
   long long M = 0;
   int maxstore = 0;
   DevBuf<double> dstore, dpp, dacc, dphi_m, dphi, drho, dphitot, dmol, v64, v32, partial2;
   DevBuf<double> adj_a, adj_b;  // VJP work arrays
+  DevBuf<double> dacc2, dpp2;   // the backward chain's, in a middle-out sweep
   void init(System& S) {
     M = S.lat.d.M;
     maxstore = S.store_slots;
@@ -44,19 +55,15 @@ struct Tangent {
     dmol.alloc((size_t)S.nm * 4);
     v64.alloc((size_t)S.ns * M);
     v32.alloc((size_t)S.ns * M);  // per-state FP32 directions (half used)
+    if (S.use_block3) {
+      dacc2.alloc((size_t)S.nm * S.ns * M);
+      dpp2.alloc(2 * (size_t)M);
+      zero<double>(dpp2.p, 2 * (long long)M, S.stream);  // halo slots are never written
+    }
     partial2.alloc((size_t)S.npart);
   }
 };
 
-template <typename T>
-static __global__ void k_zero(T* p, long long n) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = T(0);
-}
-template <typename T>
-static void zero(T* p, long long n, hipStream_t st) {
-  k_zero<T><<<grid1(n, kTB), kTB, 0, st>>>(p, n);
-}
 
 static __global__ void k_to_f32(const double* a, float* b, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -105,29 +112,48 @@ __global__ void __launch_bounds__(kTB) k_tprop(DevLattice L, const T* __restrict
                                                const T* __restrict__ Gp, const T* __restrict__ dGp,
                                                T* __restrict__ dacc, RankScale sc) {
   long long q = site_ordinal();
+  const bool in = q < L.V;
+  // this launch's loads first: the step scale below waits on the previous
+  // launch's maxima while they are in flight
+  long long r = 0;
+  T a = T(0), da = T(0), gv = T(0), vr = T(0), gpr = T(0), dgpr = T(0), dac = T(0);
+  if (in) {
+    int x, y, z;
+    interior_coords(L, q, x, y, z);
+    r = sidx(L, x, y, z);
+    a = stencil<T, KIND>(L, Gin, x, y, z);
+    da = stencil<T, KIND>(L, dGin, x, y, z);
+    gv = g[r];
+    vr = v[r];
+    if constexpr (CONNECT) {
+      gpr = Gp[r];
+      dgpr = dGp[r];
+      dac = dacc[r];
+    }
+  }
   T scale = T(1), cfac = T(1);
   double lncum = 0;
   if constexpr (SCALED) {
+    const double l0 = sc.lnv[sc.step - 1];
+    double lp = 0.0, rf = 0.0;
+    if constexpr (CONNECT) {
+      lp = sc.lnpair[sc.pair];
+      rf = *sc.ref;
+    }
     float mprev, unused;
     block_step_max2(sc.maxv, sc.step - 1, nullptr, 0, mprev, unused);
     if (!(mprev > 0.f)) mprev = 1.f;
-    lncum = sc.lnv[sc.step - 1] + log((double)mprev);
+    lncum = l0 + log((double)mprev);
     scale = (T)(1.f / mprev);
-    if constexpr (CONNECT) cfac = (T)exp(lncum + sc.lnpair[sc.pair] - *sc.ref);
+    if constexpr (CONNECT) cfac = (T)exp(lncum + lp - rf);
   }
   float m = 0.f;
-  if (q < L.V) {
-    int x, y, z;
-    interior_coords(L, q, x, y, z);
-    long long r = sidx(L, x, y, z);
-    T a = stencil<T, KIND>(L, Gin, x, y, z);
-    T da = stencil<T, KIND>(L, dGin, x, y, z);
-    T gv = g[r];
+  if (in) {
     T Go = gv * a * scale;
-    T dGo = gv * (da - v[r] * a) * scale;
+    T dGo = gv * (da - vr * a) * scale;
     Gout[r] = Go;
     dGout[r] = dGo;
-    if constexpr (CONNECT) dacc[r] += (dGo * Gp[r] + Go * dGp[r]) * cfac;
+    if constexpr (CONNECT) dacc[r] = dac + (dGo * gpr + Go * dgpr) * cfac;
     m = (float)Go;
   }
   if constexpr (SCALED) {
@@ -280,8 +306,9 @@ __global__ void k_tmol_finalize(MolParams mp, const double* p1, const double* p2
 // dphi_{m,t} at interior sites
 template <typename T>
 __global__ void k_tmol_phi(DevLattice L, const double* __restrict__ phi_m, const T* __restrict__ dacc,
-                           const double* __restrict__ g, const double* __restrict__ v, const double* __restrict__ scal,
-                           const double* __restrict__ dm, int monomer, double* __restrict__ dphi_m) {
+                           const T* __restrict__ dacc2, const double* __restrict__ g, const double* __restrict__ v,
+                           const double* __restrict__ scal, const double* __restrict__ dm, int monomer,
+                           double* __restrict__ dphi_m) {
   long long q = site_ordinal();
   if (q >= L.V) return;
   int x, y, z;
@@ -294,7 +321,7 @@ __global__ void k_tmol_phi(DevLattice L, const double* __restrict__ phi_m, const
   if (monomer)
     out = ph * (rel - v[r]);
   else
-    out = gv > 0 ? ph * (rel + v[r]) + dm[3] * (double)dacc[r] / gv : 0.0;
+    out = gv > 0 ? ph * (rel + v[r]) + dm[3] * ((double)dacc[r] + (dacc2 ? (double)dacc2[r] : 0.0)) / gv : 0.0;
   dphi_m[r] = out;
 }
 
@@ -448,33 +475,139 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
   auto bsc = [&](int step, int pair) { return RankScale{maxb, lnb, lnf, scal + MS_REF, step, pair}; };
   auto state_of_rank = [&](int s) { return mo.seq[s - 1]; };
   auto nosplit = [](int) { return false; };
-  auto launch = [&](int sa, int sb, int in_slot_or_pp, bool in_is_pp, int t, unsigned* maxv, double* lnv,
-                    float* const* outs, float* const* douts, const int* pslots, const int* pidx, bool connect) {
+  // the arguments of ranks sa..sb of a chain from (in, din); with an
+  // accumulator each rank is composed with the ranks in slots pslots[k],
+  // whose log scale is lnpair[pidx[k]]
+  auto args = [&](int sa, int sb, const float* in, const float* din, int t, unsigned* maxv, double* lnv,
+                  const double* lnpair, float* const* outs, float* const* douts, const int* pslots, const int* pidx,
+                  float* acc_t) {
     B3Args<float> a{};
-    a.in = in_is_pp ? pp[in_slot_or_pp] : slot(in_slot_or_pp);
-    a.din = in_is_pp ? dpp[in_slot_or_pp] : dslot(in_slot_or_pp);
+    a.in = in;
+    a.din = din;
     a.g = gp(t);
     a.v = vp(t);
     a.nr = sb - sa + 1;
     for (int k = 0; k < a.nr; k++) {
       a.out[k] = outs[k];
       a.dout[k] = douts[k];
-      if (connect) {
+      if (acc_t) {
         a.pair[k] = slot(pslots[k]);
         a.dpair[k] = dslot(pslots[k]);
         a.pidx[k] = pidx[k];
       }
     }
-    a.acc = connect ? daccp(t) : nullptr;
+    a.acc = acc_t;
     a.maxv = maxv;
     a.lnv = lnv;
-    a.lnpair = lnf;
+    a.lnpair = lnpair;
     a.ref = scal + MS_REF;
     a.gmax = S.d_gmax + (size_t)t * kStepWords;
     a.s0 = sa - 1;
-    const auto ev = S.prof.events(connect ? 3 : 2, SFX_FP32, a.nr);
-    launch_block3<float>(L, a, true, true, connect, st, ev.first, ev.second);
+    return a;
   };
+  auto go = [&](const B3Launch<float>& la, bool connect) {
+    int nr = 0;
+    for (int c = 0; c < la.nchains; c++) nr += la.a[c].nr;
+    const auto ev = S.prof.events(connect ? 3 : 2, SFX_FP32, nr);
+    launch_block3<float>(L, la, true, true, connect, st, ev.first, ev.second);
+  };
+  auto launch = [&](int sa, int sb, int in_slot_or_pp, bool in_is_pp, int t, unsigned* maxv, double* lnv,
+                    float* const* outs, float* const* douts, const int* pslots, const int* pidx, bool connect) {
+    B3Launch<float> la{};
+    la.a[0] = args(sa, sb, in_is_pp ? pp[in_slot_or_pp] : slot(in_slot_or_pp),
+                   in_is_pp ? dpp[in_slot_or_pp] : dslot(in_slot_or_pp), t, maxv, lnv, lnf, outs, douts, pslots,
+                   pidx, connect ? daccp(t) : nullptr);
+    la.nchains = 1;
+    go(la, connect);
+  };
+  if (S.middle_out(mo)) {
+    // the schedule of System::sweep_blocked's middle-out branch, with tangents
+    const int h = N / 2;
+    float* dacc2 = (float*)W.dacc2.p;
+    auto dacc2p = [&](int t) { return dacc2 + ((size_t)m * S.ns + t) * Mn; };
+    float* qq[2] = {(float*)W.dpp2.p, (float*)W.dpp2.p + Mn};  // backward running rank
+    float* dqq[2] = {(float*)W.dpp2.p + 2 * Mn, (float*)W.dpp2.p + 3 * Mn};
+    for (int t = 0; t < S.ns; t++)
+      if (mo.uses[t]) zero<float>(dacc2p(t), Mn, st);
+    auto rank_of = [&](int k) { return N - k + 1; };
+    auto bstate = [&](int k) { return mo.seq[rank_of(k) - 1]; };
+    k_tinit<float><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0));
+    const int tN = mo.seq[N - 1];
+    k_tinit<float><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(tN), vp(tN), slot(N - 1), dslot(N - 1), bsc(1, N));
+    const auto F1 = chain_launches(2, h, 2, state_of_rank);
+    const auto B1 = chain_launches(2, N - h, 2, bstate);
+    for (size_t i = 0; i < std::max(F1.size(), B1.size()); i++) {
+      B3Launch<float> la{};
+      if (i < F1.size()) {
+        const int sa = F1[i].first, sb = F1[i].second;
+        float* outs[kB3Ranks] = {};
+        float* douts[kB3Ranks] = {};
+        for (int s = sa; s <= sb; s++) {
+          outs[s - sa] = slot(s - 1);
+          douts[s - sa] = dslot(s - 1);
+        }
+        la.a[la.nchains++] = args(sa, sb, slot(sa - 2), dslot(sa - 2), mo.seq[sa - 1], maxf, lnf, nullptr, outs,
+                                  douts, nullptr, nullptr, nullptr);
+      }
+      if (i < B1.size()) {
+        const int ka = B1[i].first, kb = B1[i].second;
+        float* outs[kB3Ranks] = {};
+        float* douts[kB3Ranks] = {};
+        for (int k = ka; k <= kb; k++) {
+          outs[k - ka] = slot(rank_of(k) - 1);
+          douts[k - ka] = dslot(rank_of(k) - 1);
+        }
+        const int in = rank_of(ka - 1) - 1;
+        la.a[la.nchains++] = args(ka, kb, slot(in), dslot(in), bstate(ka), maxb, lnb, nullptr, outs, douts, nullptr,
+                                  nullptr, nullptr);
+      }
+      go(la, false);
+    }
+    launch_set_ref_pair(scal, lnf, h, lnb, N - h, st);
+    const auto F2 = chain_launches(h + 1, N, h + 1, state_of_rank);
+    const auto B2 = chain_launches(N - h + 1, N, N - h + 1, bstate);
+    int fc = 0, bc = 0;
+    for (size_t i = 0; i < std::max(F2.size(), B2.size()); i++) {
+      B3Launch<float> la{};
+      if (i < F2.size()) {
+        const int sa = F2[i].first, sb = F2[i].second;
+        float* outs[kB3Ranks] = {};
+        float* douts[kB3Ranks] = {};
+        int pslots[kB3Ranks] = {}, pidx[kB3Ranks] = {};
+        for (int s = sa; s <= sb; s++) {
+          pslots[s - sa] = s - 1;  // backward rank s
+          pidx[s - sa] = N - s + 1;
+        }
+        outs[sb - sa] = pp[fc ^ 1];
+        douts[sb - sa] = dpp[fc ^ 1];
+        const int t = mo.seq[sa - 1];
+        la.a[la.nchains++] = args(sa, sb, i == 0 ? slot(h - 1) : pp[fc], i == 0 ? dslot(h - 1) : dpp[fc], t, maxf, lnf,
+                                  lnb, outs, douts, pslots, pidx, daccp(t));
+        fc ^= 1;
+      }
+      if (i < B2.size()) {
+        const int ka = B2[i].first, kb = B2[i].second;
+        float* outs[kB3Ranks] = {};
+        float* douts[kB3Ranks] = {};
+        int pslots[kB3Ranks] = {}, pidx[kB3Ranks] = {};
+        for (int k = ka; k <= kb; k++) {
+          pslots[k - ka] = rank_of(k) - 1;  // forward rank
+          pidx[k - ka] = rank_of(k);
+        }
+        outs[kb - ka] = qq[bc ^ 1];
+        douts[kb - ka] = dqq[bc ^ 1];
+        const int t = bstate(ka);
+        la.a[la.nchains++] = args(ka, kb, i == 0 ? slot(h) : qq[bc], i == 0 ? dslot(h) : dqq[bc], t, maxb, lnb, lnf,
+                                  outs, douts, pslots, pidx, dacc2p(t));
+        bc ^= 1;
+      }
+      go(la, true);
+    }
+    unsigned nb = grid1(L.V, kTB);
+    k_wsum2<float><<<nb, kTB, 0, st>>>(L, pp[fc], dpp[fc], S.d_partial, W.partial2.p);  // forward rank N
+    k_tmol_finalize<<<1, 256, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, (int)nb, S.d_molscal, W.dmol.p, S.nm);
+    return;
+  }
   k_tinit<float><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0));
   {
     int prev = 0, parity = 0;
@@ -748,12 +881,16 @@ static void tmol_phi(System& S, Tangent& W, int m, const double* v64state) {
   const DevLattice& L = S.lat.d;
   const long long Mn = L.M;
   Molecule& mo = S.mols[m];
+  // the FP32 blocked tangent sweep of a middle-out chain leaves half of the
+  // composition in the second accumulator
+  const bool two = sizeof(T) == 4 && S.middle_out(mo);
   for (int t = 0; t < S.ns; t++) {
     if (!mo.uses[t]) continue;
     size_t off = ((size_t)m * S.ns + t) * Mn;
     k_tmol_phi<T><<<grid1(L.V, kTB), kTB, 0, S.stream>>>(
-        L, S.d_phi_m + off, (const T*)W.dacc.p + off, S.d_swf + (size_t)t * Mn, v64state + (size_t)t * Mn,
-        S.d_molscal + (size_t)m * MS_COUNT, W.dmol.p + m * 4, mo.kind == 0, W.dphi_m.p + off);
+        L, S.d_phi_m + off, (const T*)W.dacc.p + off, two ? (const T*)W.dacc2.p + off : nullptr,
+        S.d_swf + (size_t)t * Mn, v64state + (size_t)t * Mn, S.d_molscal + (size_t)m * MS_COUNT, W.dmol.p + m * 4,
+        mo.kind == 0, W.dphi_m.p + off);
   }
 }
 

@@ -2,6 +2,9 @@
 // lattice (temporal blocking through LDS tiles).
 #pragma once
 
+#include <utility>
+#include <vector>
+
 #include "sfx_kernels.h"
 
 namespace sfx {
@@ -33,6 +36,15 @@ struct B3Args {
   int pidx[kB3Ranks];
 };
 
+// One launch advancing up to two independent chains (the forward and the
+// backward chain of a molecule): workgroups [0, nb) take chain 0, [nb, 2 nb)
+// chain 1. Both run the same kernel variant (tangent / scaled / connect).
+template <typename T>
+struct B3Launch {
+  B3Args<T> a[2];
+  int nchains;
+};
+
 // Splits chain steps a..b (ascending) into launches: a launch ends at a grid
 // line ((step - g0 + 1) % kB3Ranks == 0), before a step of another segment
 // state, or before a step that split() marks. f(first, last) per launch. The
@@ -49,6 +61,14 @@ void chain_blocks(int a, int b, int g0, State state, Split split, F f) {
   }
 }
 
+// the launches of chain_blocks(a, b, g0, state, no extra splits) as a list
+template <class State>
+std::vector<std::pair<int, int>> chain_launches(int a, int b, int g0, State state) {
+  std::vector<std::pair<int, int>> out;
+  chain_blocks(a, b, g0, state, [](int) { return false; }, [&](int s, int e) { out.emplace_back(s, e); });
+  return out;
+}
+
 // true when the blocked sweep applies: 3-D simple cubic, first order, every
 // side at least kB3Ranks + 1 layers (one reflection reaches the interior)
 bool block3_ok(const DevLattice& L);
@@ -56,6 +76,9 @@ bool block3_ok(const DevLattice& L);
 // ev0/ev1 (optional): timing events the dispatch records at kernel start and end
 template <typename T>
 void launch_block3(const DevLattice& L, const B3Args<T>& a, bool tangent, bool scaled, bool connect,
+                   hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+template <typename T>
+void launch_block3(const DevLattice& L, const B3Launch<T>& la, bool tangent, bool scaled, bool connect,
                    hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 // max over the interior of each state's FP32 SWF, into step-max slots

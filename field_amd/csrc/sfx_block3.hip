@@ -64,7 +64,11 @@ __device__ __forceinline__ T block_max_T(T m, float* red) {
 }
 
 template <typename T, bool TANGENT, bool SCALED, bool CONNECT>
-__global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
+__global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Launch<T> la, int nb) {
+  // chain of this workgroup and its tile number within that chain's grid
+  const int chain = (int)blockIdx.x >= nb ? 1 : 0;
+  const int bl = (int)blockIdx.x - chain * nb;
+  const B3Args<T>& a = la.a[chain];
   extern __shared__ __align__(16) unsigned char smem[];
   T* G0 = (T*)smem;
   T* G1 = G0 + kR;
@@ -76,7 +80,9 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
 
   const int t = threadIdx.x;
   const int nbz = (L.nz + kTZ - 1) / kTZ, nby = (L.ny + kTY - 1) / kTY;
-  int b = blockIdx.x;
+  // XCD-aware: each XCD takes a contiguous run of tiles (x-slabs), so the y / z
+  // halos its tiles share stay in that XCD's L2
+  int b = (int)xcd_tile(bl, nb);
   const int bz = b % nbz;
   b /= nbz;
   const int by = b % nby;
@@ -89,6 +95,52 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
   const bool owner = x0 + oX <= L.nx && y0 + oY <= L.ny && z0 + oZ <= L.nz;
   const long long ogi = owner ? sidx(L, x0 + oX, y0 + oY, z0 + oZ) : 0;
   const int oidx = (kH + oX) * kRYZ + (kH + oY) * kRZ + (kH + oZ);
+
+  // the logs the scale factors build on were written by earlier launches: load
+  // them now, so their latency overlaps the region loads below
+  const int lane = t & 63;
+  double lnv0 = 0.0, lnp = 0.0, lref = 0.0;
+  if constexpr (SCALED) {
+    lnv0 = a.lnv[a.s0];
+    if constexpr (CONNECT) {
+      int pi = 0;
+#pragma unroll
+      for (int k = 0; k < kB3Ranks; k++)
+        if (lane == k) pi = a.pidx[k];
+      if (lane < nr) lnp = a.lnpair[pi];
+      lref = *a.ref;
+    }
+  }
+
+  // issue the loads of rank s0 (and its tangent), g (and v) on the tile widened
+  // by nr first: the scale factors below are computed while they are in flight,
+  // and no LDS store waits on them before that
+  constexpr int kIt = (kR + kB3 - 1) / kB3;
+  const int wlo = kH - nr, wy = kTY + 2 * nr, wz = kTZ + 2 * nr;
+  const int nreg = (kTX + 2 * nr) * wy * wz;
+  T vg[kIt], vgg[kIt], vd[kIt], vv[kIt];
+  int li[kIt];
+#pragma unroll
+  for (int j = 0; j < kIt; j++) {
+    const int i = t + j * kB3;
+    li[j] = -1;
+    if (i < nreg) {
+      const int Z = i % wz, r = i / wz, Y = r % wy, X = r / wy;
+      const int x = extend(x0 + wlo + X - kH, L.nx, L.periodic[0], L.mirror2_1[0], L.mirror2_2[0]);
+      const int y = extend(y0 + wlo + Y - kH, L.ny, L.periodic[1], L.mirror2_1[1], L.mirror2_2[1]);
+      const int z = extend(z0 + wlo + Z - kH, L.nz, L.periodic[2], L.mirror2_1[2], L.mirror2_2[2]);
+      const long long gi = sidx(L, x, y, z);
+      li[j] = (wlo + X) * kRYZ + (wlo + Y) * kRZ + (wlo + Z);
+      vg[j] = a.in[gi];
+      vgg[j] = a.g[gi];
+      if constexpr (TANGENT) {
+        vd[j] = a.din[gi];
+        vv[j] = a.v[gi];
+      }
+    }
+  }
+
+  // the partner ranks are needed only from the first composition on
   T pr[kB3Ranks], dpr[kB3Ranks];
   if constexpr (CONNECT) {
 #pragma unroll
@@ -107,7 +159,6 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
 #pragma unroll
   for (int k = 0; k < kB3Ranks; k++) cfac[k] = T(1);
   if constexpr (SCALED) {
-    const int lane = t & 63;
     float m0, gm;
     block_step_max2(a.maxv, a.s0, a.gmax, 0, m0, gm);
     if (!(m0 > 0.f)) m0 = 1.f;
@@ -115,55 +166,23 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
     in_scale = (T)(1.f / m0);
     ginv = (T)(1.f / gm);
     const double lv = lane == 0 ? log((double)m0) : (lane == 1 ? log((double)gm) : 0.0);
-    ln_base = a.lnv[a.s0] + __shfl(lv, 0, 64);
+    ln_base = lnv0 + __shfl(lv, 0, 64);
     lng = __shfl(lv, 1, 64);
     if constexpr (CONNECT) {
-      int pi = 0;
-#pragma unroll
-      for (int k = 0; k < kB3Ranks; k++)
-        if (lane == k) pi = a.pidx[k];
-      const double c = lane < nr ? exp(ln_base + (lane + 1) * lng + a.lnpair[pi] - *a.ref) : 1.0;
+      const double c = lane < nr ? exp(ln_base + (lane + 1) * lng + lnp - lref) : 1.0;
 #pragma unroll
       for (int k = 0; k < kB3Ranks; k++) cfac[k] = (T)__shfl(c, k, 64);
     }
   }
 
-  // load rank s0 (and its tangent), g (and v) on the tile widened by nr: all
-  // of a thread's HBM loads are issued before any LDS store waits on them
-  {
-    constexpr int kIt = (kR + kB3 - 1) / kB3;
-    const int lo = kH - nr, w = kTX + 2 * nr, wy = kTY + 2 * nr, wz = kTZ + 2 * nr;
-    const int n = w * wy * wz;
-    T vg[kIt], vgg[kIt], vd[kIt], vv[kIt];
-    int li[kIt];
 #pragma unroll
-    for (int j = 0; j < kIt; j++) {
-      const int i = t + j * kB3;
-      li[j] = -1;
-      if (i < n) {
-        const int Z = i % wz, r = i / wz, Y = r % wy, X = r / wy;
-        const int x = extend(x0 + lo + X - kH, L.nx, L.periodic[0], L.mirror2_1[0], L.mirror2_2[0]);
-        const int y = extend(y0 + lo + Y - kH, L.ny, L.periodic[1], L.mirror2_1[1], L.mirror2_2[1]);
-        const int z = extend(z0 + lo + Z - kH, L.nz, L.periodic[2], L.mirror2_1[2], L.mirror2_2[2]);
-        const long long gi = sidx(L, x, y, z);
-        li[j] = (lo + X) * kRYZ + (lo + Y) * kRZ + (lo + Z);
-        vg[j] = a.in[gi];
-        vgg[j] = a.g[gi];
-        if constexpr (TANGENT) {
-          vd[j] = a.din[gi];
-          vv[j] = a.v[gi];
-        }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kIt; j++) {
-      if (li[j] >= 0) {
-        G0[li[j]] = vg[j] * in_scale;
-        gl[li[j]] = vgg[j];
-        if constexpr (TANGENT) {
-          D0[li[j]] = vd[j] * in_scale;
-          vl[li[j]] = vv[j];
-        }
+  for (int j = 0; j < kIt; j++) {
+    if (li[j] >= 0) {
+      G0[li[j]] = vg[j] * in_scale;
+      gl[li[j]] = vgg[j];
+      if constexpr (TANGENT) {
+        D0[li[j]] = vd[j] * in_scale;
+        vl[li[j]] = vv[j];
       }
     }
   }
@@ -236,7 +255,7 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Args<T> a) {
     const T m = block_max_T<T>(mloc, red);
     if (t == 0) {
       put_step_max(a.maxv, a.s0 + nr, (float)m);
-      if (blockIdx.x == 0)
+      if (bl == 0)
         for (int k = 1; k <= nr; k++) a.lnv[a.s0 + k] = ln_base + k * lng;
     }
   }
@@ -248,7 +267,7 @@ static size_t b3_lds() {
 }
 
 template <typename T, bool TG, bool SC, bool CN>
-static void b3_launch(const DevLattice& L, const B3Args<T>& a, unsigned nb, hipStream_t st, hipEvent_t ev0,
+static void b3_launch(const DevLattice& L, const B3Launch<T>& a, unsigned nb, hipStream_t st, hipEvent_t ev0,
                       hipEvent_t ev1) {
   static bool attr = false;  // dynamic LDS above the default limit needs the attribute once
   const size_t lds = b3_lds<T, TG>();
@@ -258,13 +277,25 @@ static void b3_launch(const DevLattice& L, const B3Args<T>& a, unsigned nb, hipS
     attr = true;
   }
   // with events: recorded by the dispatch itself at kernel start and end
-  hipExtLaunchKernelGGL(k_block3<T, TG, SC, CN>, dim3(nb), dim3(kB3), (uint32_t)lds, st, ev0, ev1, 0u, L, a);
+  hipExtLaunchKernelGGL(k_block3<T, TG, SC, CN>, dim3(nb * (unsigned)a.nchains), dim3(kB3), (uint32_t)lds, st, ev0,
+                        ev1, 0u, L, a, (int)nb);
 }
 
 template <typename T>
 void launch_block3(const DevLattice& L, const B3Args<T>& a, bool tangent, bool scaled, bool connect,
                    hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-  SFX_REQUIRE(a.nr >= 1 && a.nr <= kB3Ranks, SFX_E_INVALID, "blocked launch of 1..4 ranks");
+  B3Launch<T> la{};
+  la.a[0] = a;
+  la.nchains = 1;
+  launch_block3<T>(L, la, tangent, scaled, connect, st, ev0, ev1);
+}
+
+template <typename T>
+void launch_block3(const DevLattice& L, const B3Launch<T>& a, bool tangent, bool scaled, bool connect,
+                   hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+  SFX_REQUIRE(a.nchains == 1 || a.nchains == 2, SFX_E_INVALID, "a blocked launch advances one or two chains");
+  for (int c = 0; c < a.nchains; c++)
+    SFX_REQUIRE(a.a[c].nr >= 1 && a.a[c].nr <= kB3Ranks, SFX_E_INVALID, "blocked launch of 1..4 ranks");
   SFX_REQUIRE(!(tangent && sizeof(T) == 8), SFX_E_UNSUPPORTED, "FP64 tangent sweeps run rank by rank");
   const size_t lds_base = b3_lds<T, false>(), lds_tan = b3_lds<T, true>();
   SFX_REQUIRE((lds_base <= 160 * 1024 && (sizeof(T) == 8 || lds_tan <= 160 * 1024)), SFX_E_UNSUPPORTED,
@@ -323,6 +354,10 @@ void launch_gmax(const DevLattice& L, const float* swf32, int ns, unsigned* gmax
 template void launch_block3<float>(const DevLattice&, const B3Args<float>&, bool, bool, bool, hipStream_t,
                                    hipEvent_t, hipEvent_t);
 template void launch_block3<double>(const DevLattice&, const B3Args<double>&, bool, bool, bool, hipStream_t,
+                                    hipEvent_t, hipEvent_t);
+template void launch_block3<float>(const DevLattice&, const B3Launch<float>&, bool, bool, bool, hipStream_t,
+                                   hipEvent_t, hipEvent_t);
+template void launch_block3<double>(const DevLattice&, const B3Launch<double>&, bool, bool, bool, hipStream_t,
                                     hipEvent_t, hipEvent_t);
 
 }  // namespace sfx

@@ -102,9 +102,14 @@ template <typename T>
 int launch_wsum(const DevLattice& L, const T* G, double* partial, hipStream_t st);
 void launch_mol_finalize(const MolParams& mp, const double* partial, int nparts, double* scal,
                          const double* lnfN, const double* allscal, int nmol, hipStream_t st);
+// phi_m = pref (acc + acc2) / g; acc2 (the backward chain's accumulator of a
+// middle-out sweep) may be null
 template <typename A>
 void launch_mol_phi(const DevLattice& L, const A* acc, const double* g, const double* scal, int monomer,
-                    double* phi_m, hipStream_t st);
+                    double* phi_m, hipStream_t st, const A* acc2 = nullptr);
+// scal[MS_REF] = lna[ia] + lnb[ib] (0 when lna is null): the reference log of
+// a middle-out sweep's compositions
+void launch_set_ref_pair(double* scal, const double* lna, int ia, const double* lnb, int ib, hipStream_t st);
 void launch_swf(const DevLattice& L, const double* u, const unsigned char* itmask, int nstates,
                 const int* st_itvar, double* swf, float* swf32, hipStream_t st);
 void launch_phi_total(const DevLattice& L, int nstates, int nmol, const double* phi_m, const int* mol_uses,

@@ -94,10 +94,11 @@ __global__ void k_init_rank(DevLattice L, const T* __restrict__ g, T* __restrict
 
 // scale factor of step `step`: out values are multiplied by 1/max(previous step)
 __device__ __forceinline__ float step_scale(const RankScale& sc, double& lncum) {
+  const double l0 = sc.lnv[sc.step - 1];  // issued before the wait on the maxima
   float mprev, unused;
   block_step_max2(sc.maxv, sc.step - 1, nullptr, 0, mprev, unused);
   if (!(mprev > 0.f)) mprev = 1.f;  // all-zero propagator: keep it unscaled
-  lncum = sc.lnv[sc.step - 1] + log((double)mprev);
+  lncum = l0 + log((double)mprev);
   return 1.f / mprev;
 }
 
@@ -108,22 +109,39 @@ __global__ void __launch_bounds__(kBlock) k_prop(DevLattice L, const T* __restri
                                                  const T* __restrict__ Gpair, A* __restrict__ acc,
                                                  RankScale sc) {
   long long q = site_ordinal();
+  const bool in = q < L.V;
+  // this launch's loads first: the step scale below waits on the previous
+  // launch's maxima while they are in flight
+  long long r = 0;
+  T v = T(0), gpv = T(0);
+  A a0 = A(0);
+  if (in) {
+    int x, y, z;
+    interior_coords(L, q, x, y, z);
+    r = sidx(L, x, y, z);
+    v = g[r] * stencil<T, KIND>(L, Gin, x, y, z);
+    if constexpr (CONNECT) {
+      gpv = Gpair[r];
+      a0 = acc[r];
+    }
+  }
   T scale = T(1);
   double lncum = 0;
   T cfac = T(1);
   if constexpr (SCALED) {
+    double lp = 0.0, rf = 0.0;
+    if constexpr (CONNECT) {
+      lp = sc.lnpair[sc.pair];
+      rf = *sc.ref;
+    }
     scale = (T)step_scale(sc, lncum);
-    if constexpr (CONNECT) cfac = (T)exp(lncum + sc.lnpair[sc.pair] - *sc.ref);
+    if constexpr (CONNECT) cfac = (T)exp(lncum + lp - rf);
   }
   float m = 0.f;
-  if (q < L.V) {
-    int x, y, z;
-    interior_coords(L, q, x, y, z);
-    long long r = sidx(L, x, y, z);
-    T v = g[r] * stencil<T, KIND>(L, Gin, x, y, z);
+  if (in) {
     if constexpr (SCALED) v *= scale;
     Gout[r] = v;
-    if constexpr (CONNECT) acc[r] += (A)(v * Gpair[r] * cfac);
+    if constexpr (CONNECT) acc[r] = a0 + (A)(v * gpv * cfac);
     m = (float)v;
   }
   if constexpr (SCALED) {
@@ -223,8 +241,9 @@ __global__ void k_mol_finalize(MolParams mp, const double* __restrict__ partial,
 
 // phi_{m,t} = pref_m * acc_{m,t} / g_t (g>0, else 0); monomers phi = pref * g.
 template <typename A>
-__global__ void k_mol_phi(DevLattice L, const A* __restrict__ acc, const double* __restrict__ g,
-                          const double* __restrict__ scal, int monomer, double* __restrict__ phi_m) {
+__global__ void k_mol_phi(DevLattice L, const A* __restrict__ acc, const A* __restrict__ acc2,
+                          const double* __restrict__ g, const double* __restrict__ scal, int monomer,
+                          double* __restrict__ phi_m) {
   long long q = site_ordinal();
   if (q >= L.V) return;
   int x, y, z;
@@ -236,8 +255,16 @@ __global__ void k_mol_phi(DevLattice L, const A* __restrict__ acc, const double*
   if (monomer)
     v = pref * gv;
   else
-    v = gv > 0 ? ((double)acc[r] / gv) * pref : 0.0;
+    v = gv > 0 ? (((double)acc[r] + (acc2 ? (double)acc2[r] : 0.0)) / gv) * pref : 0.0;
   phi_m[r] = v;
+}
+
+__global__ void k_set_ref_pair(double* scal, const double* lna, int ia, const double* lnb, int ib) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) scal[MS_REF] = lna ? lna[ia] + lnb[ib] : 0.0;
+}
+
+void launch_set_ref_pair(double* scal, const double* lna, int ia, const double* lnb, int ib, hipStream_t st) {
+  k_set_ref_pair<<<1, 1, 0, st>>>(scal, lna, ia, lnb, ib);
 }
 
 // state totals over molecules + frozen profiles, halos from their sources;
@@ -477,8 +504,8 @@ void launch_mol_finalize(const MolParams& mp, const double* partial, int nparts,
 
 template <typename A>
 void launch_mol_phi(const DevLattice& L, const A* acc, const double* g, const double* scal, int monomer,
-                    double* phi_m, hipStream_t st) {
-  k_mol_phi<A><<<grid1(L.V, kBlock), kBlock, 0, st>>>(L, acc, g, scal, monomer, phi_m);
+                    double* phi_m, hipStream_t st, const A* acc2) {
+  k_mol_phi<A><<<grid1(L.V, kBlock), kBlock, 0, st>>>(L, acc, acc2, g, scal, monomer, phi_m);
 }
 
 void launch_swf(const DevLattice& L, const double* u, const unsigned char* itmask, int nstates,
@@ -533,8 +560,8 @@ template void launch_copy_interior<float>(const DevLattice&, const float*, float
 template int launch_wsum<double>(const DevLattice&, const double*, double*, hipStream_t);
 template int launch_wsum<float>(const DevLattice&, const float*, double*, hipStream_t);
 template void launch_mol_phi<double>(const DevLattice&, const double*, const double*, const double*, int, double*,
-                                     hipStream_t);
+                                     hipStream_t, const double*);
 template void launch_mol_phi<float>(const DevLattice&, const float*, const double*, const double*, int, double*,
-                                    hipStream_t);
+                                    hipStream_t, const float*);
 
 }  // namespace sfx

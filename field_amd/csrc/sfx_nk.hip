@@ -26,9 +26,15 @@ void jvp_at_base(System& S, const double* v, double* jv, int precision);
 void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
   const long long n = (long long)S.n_itvar * S.lat.d.M;
   const int m = std::max(1, std::min(o.gmres_restart, 30));
-  VecOps V;
-  V.init(n, S.stream);
-  DevBuf<double> basis((size_t)(m + 1) * n), w(n), dx(n), x0(n), r(n), pz(n);
+  SolveWork& W = solve_work(S, n, (size_t)m + 6);
+  VecOps& V = W.V;
+  double* const wv = W.vec(m + 1);  // Arnoldi work vector
+  double* const dx = W.vec(m + 2);
+  double* const x0 = W.vec(m + 3);
+  double* const r = W.vec(m + 4);
+  double* const pz = W.vec(m + 5);
+  double* const gs = W.gs.p;  // projections of both Gram-Schmidt passes, then ||w||^2
+  constexpr int kGsPass = 32;
   double* x = S.d_u;
   Precond* pc = nullptr;
   if (o.precondition && precond_supported(S)) {
@@ -42,14 +48,14 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
   auto op = [&](const double* v, double* out) {
     const double* a = v;
     if (pc) {
-      precond_apply(pc, S, v, pz.p, sigma);
-      a = pz.p;
+      precond_apply(pc, S, v, pz, sigma);
+      a = pz;
     }
     jvp_at_base(S, a, out, o.jvp_precision);
     if (sigma != 0.0) V.axpby(sigma, a, 1.0, out);
     st.jvp_evals++;
   };
-  auto Vk = [&](int k) { return basis.p + (size_t)k * n; };
+  auto Vk = [&](int k) { return W.vec((size_t)k); };
   hipEvent_t e0, e1;
   SFX_HIP(hipEventCreate(&e0));
   SFX_HIP(hipEventCreate(&e1));
@@ -62,8 +68,8 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
       st.residual_evals_fp32++;
     else
       st.residual_evals++;
-    V.copy(S.d_f, r.p);
-    return std::sqrt(V.dot(r.p, r.p));
+    V.copy(S.d_f, r);
+    return std::sqrt(V.dot(r, r));
   };
   double rn = eval(prec);
   int it = 0;
@@ -83,56 +89,57 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     it++;
     // ---- GMRES(m) for J dx = -r, dx0 = 0 ----
     double eta = o.gmres_rtol;
-    V.axpby(0.0, r.p, 0.0, dx.p);  // dx = 0
+    V.axpby(0.0, r, 0.0, dx);  // dx = 0
     int total = 0;
     double beta0 = rn;
     for (int cycle = 0; cycle < 1 + o.gmres_max_iter / m && total < o.gmres_max_iter; cycle++) {
       // residual of the linear system: r_l = -r - J dx
       double beta;
       if (cycle == 0) {
-        V.axpby(-1.0 / rn, r.p, 0.0, Vk(0));
+        V.axpby(-1.0 / rn, r, 0.0, Vk(0));
         beta = rn;
       } else {
-        op(dx.p, w.p);
-        V.axpby(-1.0, r.p, -1.0, w.p);  // w = -r - J P z
-        beta = std::sqrt(V.dot(w.p, w.p));
+        op(dx, wv);
+        V.axpby(-1.0, r, -1.0, wv);  // w = -r - J P z
+        beta = std::sqrt(V.dot(wv, wv));
         if (beta <= eta * beta0) break;
-        V.axpby(1.0 / beta, w.p, 0.0, Vk(0));
+        V.axpby(1.0 / beta, wv, 0.0, Vk(0));
       }
       std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0);
       g[0] = beta;
       int j = 0;
       double res = beta;
       for (; j < m && total < o.gmres_max_iter; j++) {
-        op(Vk(j), w.p);
+        op(Vk(j), wv);
         total++;
-        // classical Gram-Schmidt, twice
-        std::vector<double> h(j + 1, 0.0);
+        // classical Gram-Schmidt, twice, with the projections kept on the
+        // device; one read-back per Arnoldi step brings both passes and ||w||^2
+        std::vector<double> h(j + 1, 0.0), hb(2 * kGsPass + 1);
         for (int pass = 0; pass < 2; pass++) {
-          DotList dl{};
-          std::vector<double> hh(j + 1);
           for (int base = 0; base <= j; base += 16) {
-            int cnt = std::min(16, j + 1 - base);
-            dl.count = cnt;
-            for (int i = 0; i < cnt; i++) {
+            DotList dl{};
+            dl.count = std::min(16, j + 1 - base);
+            for (int i = 0; i < dl.count; i++) {
               dl.a[i] = Vk(base + i);
-              dl.b[i] = w.p;
+              dl.b[i] = wv;
             }
-            V.dots(dl, hh.data() + base);
+            V.dots_dev(dl, gs + pass * kGsPass + base);
           }
           VecList vl{};
-          std::vector<double> c(j + 2);
           vl.count = j + 2;
-          for (int i = 0; i <= j; i++) {
-            vl.v[i] = Vk(i);
-            c[i] = -hh[i];
-            h[i] += hh[i];
-          }
-          vl.v[j + 1] = w.p;
-          c[j + 1] = 1.0;
-          V.combine(vl, c.data(), w.p);
+          for (int i = 0; i <= j; i++) vl.v[i] = Vk(i);
+          vl.v[j + 1] = wv;
+          V.subtract_projections(vl, gs + pass * kGsPass, wv);
         }
-        double hn = std::sqrt(V.dot(w.p, w.p));
+        {
+          DotList dl{};
+          dl.count = 1;
+          dl.a[0] = dl.b[0] = wv;
+          V.dots_dev(dl, gs + 2 * kGsPass);
+        }
+        V.fetch(gs, 2 * kGsPass + 1, hb.data());
+        for (int i = 0; i <= j; i++) h[i] = (0.0 + hb[i]) + hb[kGsPass + i];
+        double hn = std::sqrt(hb[2 * kGsPass]);
         for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = h[i];
         H[(size_t)(j + 1) * m + j] = hn;
         // apply previous rotations, then a new one
@@ -150,7 +157,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
         g[j + 1] = -sn[j] * g[j];
         g[j] = cs[j] * g[j];
         res = std::fabs(g[j + 1]);
-        if (hn > 0) V.axpby(1.0 / hn, w.p, 0.0, Vk(j + 1));
+        if (hn > 0) V.axpby(1.0 / hn, wv, 0.0, Vk(j + 1));
         if (res <= eta * beta0 || hn == 0) {
           j++;
           break;
@@ -165,28 +172,27 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
         y[i] = s / H[(size_t)i * m + i];
       }
       VecList vl{};
-      std::vector<double> c(k + 1);
       vl.count = k + 1;
       for (int i = 0; i < k; i++) {
         vl.v[i] = Vk(i);
-        c[i] = y[i];
+        vl.c[i] = y[i];
       }
-      vl.v[k] = dx.p;
-      c[k] = 1.0;
-      V.combine(vl, c.data(), dx.p);
+      vl.v[k] = dx;
+      vl.c[k] = 1.0;
+      V.combine(vl, dx);
       if (o.verbose) std::fprintf(stderr, "  gmres cycle=%d its=%d est=%.3e\n", cycle, j, res / beta0);
       if (res <= eta * beta0) break;
     }
     if (pc) {  // dx = P z
-      V.copy(dx.p, w.p);
-      precond_apply(pc, S, w.p, dx.p, sigma);
+      V.copy(dx, wv);
+      precond_apply(pc, S, wv, dx, sigma);
     }
     if (ptc) {
-      V.copy(x, x0.p);
-      V.axpby(1.0, dx.p, 1.0, x);
+      V.copy(x, x0);
+      V.axpby(1.0, dx, 1.0, x);
       const double rt = eval(prec);
       if (!std::isfinite(rt) || rt > 5.0 * rn) {
-        V.copy(x0.p, x);
+        V.copy(x0, x);
         rn = eval(prec);  // back to the last point (and its base state for the JVPs)
         dt *= 0.25;
         if (o.verbose) std::fprintf(stderr, "  ptc step rejected (|f|=%.3e), dt=%.3e\n", rt, dt);
@@ -200,12 +206,12 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
       continue;
     }
     // ---- backtracking line search on ||f|| ----
-    V.copy(x, x0.p);
+    V.copy(x, x0);
     double alpha = 1.0, rt = 0, best = INFINITY, best_alpha = 0;
     bool accepted = false;
     for (int ls = 0; ls < 12; ls++) {
-      V.copy(x0.p, x);
-      V.axpby(alpha, dx.p, 1.0, x);
+      V.copy(x0, x);
+      V.axpby(alpha, dx, 1.0, x);
       rt = eval(prec);
       if (std::isfinite(rt) && rt < best) {
         best = rt;
@@ -219,9 +225,9 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     }
     if (!accepted) {
       // take the best finite trial if it lowers ||f|| at all, else stop at x0
-      V.copy(x0.p, x);
+      V.copy(x0, x);
       if (best < rn) {
-        V.axpby(best_alpha, dx.p, 1.0, x);
+        V.axpby(best_alpha, dx, 1.0, x);
         rt = eval(prec);
       } else {
         rt = eval(prec);

@@ -108,12 +108,29 @@ __device__ __forceinline__ void curved_fractions(const DevLattice& L, int x, T& 
 
 // the forward partner G_f(s) at site r (and its tangent), recovered from the
 // stored W_f(s-1) exactly as the forward step produced it: g(r) times the
-// forward step's scale times the gathered row; rank 1 (Wp null) is g itself
+// forward step's scale times the gathered row; rank 1 (Wp null) is g itself.
+// pair_rows gathers the rows (the loads), pair_scale applies g and the scale,
+// so a kernel can issue the loads before it waits on the scale.
 template <typename T, bool TANGENT, bool CURVED>
-__device__ __forceinline__ void pair_rank(const DevLattice& L, const T* __restrict__ Wp, const T* __restrict__ dWp,
-                                          T graw, T vv, long long M, int x, const long long nbr[6],
-                                          const bool fl[6], long long r, T pscale, T gp[6], T dgp[6]) {
-  if (!Wp) {
+__device__ __forceinline__ void pair_rows(const DevLattice& L, const T* __restrict__ Wp, const T* __restrict__ dWp,
+                                          long long M, int x, const long long nbr[6], const bool fl[6], long long r,
+                                          T pa[6], T dpa[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; k++) pa[k] = dpa[k] = T(0);
+  if (!Wp) return;
+  T up, dn;
+  curved_fractions<T, CURVED>(L, x, up, dn);
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    pa[k] = fwd_row<T, CURVED>(Wp, M, nbr, fl, r, k, up, dn);
+    if constexpr (TANGENT) dpa[k] = fwd_row<T, CURVED>(dWp, M, nbr, fl, r, k, up, dn);
+  }
+}
+
+template <typename T, bool TANGENT>
+__device__ __forceinline__ void pair_scale(bool rank1, T graw, T vv, T pscale, const T pa[6], const T dpa[6], T gp[6],
+                                           T dgp[6]) {
+  if (rank1) {
 #pragma unroll
     for (int k = 0; k < 6; k++) {
       gp[k] = graw;
@@ -122,17 +139,20 @@ __device__ __forceinline__ void pair_rank(const DevLattice& L, const T* __restri
     return;
   }
   const T gvp = graw * pscale;
-  T up, dn;
-  curved_fractions<T, CURVED>(L, x, up, dn);
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    const T a = fwd_row<T, CURVED>(Wp, M, nbr, fl, r, k, up, dn);
-    gp[k] = gvp * a;
-    if constexpr (TANGENT) {
-      const T da = fwd_row<T, CURVED>(dWp, M, nbr, fl, r, k, up, dn);
-      dgp[k] = gvp * (da - vv * a);
-    }
+    gp[k] = gvp * pa[k];
+    if constexpr (TANGENT) dgp[k] = gvp * (dpa[k] - vv * pa[k]);
   }
+}
+
+template <typename T, bool TANGENT, bool CURVED>
+__device__ __forceinline__ void pair_rank(const DevLattice& L, const T* __restrict__ Wp, const T* __restrict__ dWp,
+                                          T graw, T vv, long long M, int x, const long long nbr[6],
+                                          const bool fl[6], long long r, T pscale, T gp[6], T dgp[6]) {
+  T pa[6], dpa[6];
+  pair_rows<T, TANGENT, CURVED>(L, Wp, dWp, M, x, nbr, fl, r, pa, dpa);
+  pair_scale<T, TANGENT>(!Wp, graw, vv, pscale, pa, dpa, gp, dgp);
 }
 
 // forward: G^k_s(r) = g(r) (T G_{s-1})^k(r - e_k) from the stored W_{s-1};
@@ -143,38 +163,51 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
                                                 T* __restrict__ Wout, T* __restrict__ dWout, T S, RankScale sc) {
   long long q = site_ordinal();
   const long long M = L.M;
-  T scale = T(1);
-  double lncum = 0;
-  if constexpr (SCALED) {
-    float mp, unused;
-    block_step_max2(sc.maxv, sc.step - 1, nullptr, 0, mp, unused);
-    if (!(mp > 0.f)) mp = 1.f;
-    lncum = sc.lnv[sc.step - 1] + log((double)mp);
-    scale = (T)(1.f / mp);
-  }
-  const T Lw = (T(1) - S) / T(4);
-  float m = 0.f;
-  if (q < L.V) {
+  const bool in = q < L.V;
+  // this launch's loads first: the step scale below waits on the previous
+  // launch's maxima while they are in flight
+  long long r = 0;
+  T graw = T(0), vv = T(0);
+  T a[6], da[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) a[k] = da[k] = T(0);
+  if (in) {
     int x, y, z;
     interior_coords(L, q, x, y, z);
-    const long long r = sidx(L, x, y, z);
-    const T gv = g[r] * scale;
-    const T vv = TANGENT ? v[r] : T(0);
+    r = sidx(L, x, y, z);
+    graw = g[r];
+    if constexpr (TANGENT) vv = v[r];
     T up, dn;
     curved_fractions<T, CURVED>(L, x, up, dn);
     long long nbr[6];
     bool fl[6];
     neighbours(L, x, y, z, nbr, fl);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      a[k] = fwd_row<T, CURVED>(Win, M, nbr, fl, r, k, up, dn);
+      if constexpr (TANGENT) da[k] = fwd_row<T, CURVED>(dWin, M, nbr, fl, r, k, up, dn);
+    }
+  }
+  T scale = T(1);
+  double lncum = 0;
+  if constexpr (SCALED) {
+    const double l0 = sc.lnv[sc.step - 1];
+    float mp, unused;
+    block_step_max2(sc.maxv, sc.step - 1, nullptr, 0, mp, unused);
+    if (!(mp > 0.f)) mp = 1.f;
+    lncum = l0 + log((double)mp);
+    scale = (T)(1.f / mp);
+  }
+  const T Lw = (T(1) - S) / T(4);
+  float m = 0.f;
+  if (in) {
+    const T gv = graw * scale;
     T G[6], dG[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-      const T a = fwd_row<T, CURVED>(Win, M, nbr, fl, r, k, up, dn);
-      G[k] = gv * a;
+      G[k] = gv * a[k];
       m = fmaxf(m, (float)G[k]);
-      if constexpr (TANGENT) {
-        const T da = fwd_row<T, CURVED>(dWin, M, nbr, fl, r, k, up, dn);
-        dG[k] = gv * (da - vv * a);
-      }
+      if constexpr (TANGENT) dG[k] = gv * (da[k] - vv * a[k]);
     }
     T w[6];
     transfer_rows(S, Lw, G, w);
@@ -210,24 +243,20 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
                                                 T* __restrict__ acc, T S, RankScale sc) {
   long long q = site_ordinal();
   const long long M = L.M;
-  T scale = T(1), cfac = T(1), pscale = T(1);
-  double lncum = 0;
-  if constexpr (SCALED) {
-    float mp, mpp;
-    block_step_max2(sc.maxv, sc.step - 1, Wp ? sc.pmaxv : nullptr, sc.pair - 1, mp, mpp);
-    if (!(mp > 0.f)) mp = 1.f;
-    if (!(mpp > 0.f)) mpp = 1.f;
-    lncum = sc.lnv[sc.step - 1] + log((double)mp);
-    scale = (T)(1.f / mp);
-    pscale = (T)(1.f / mpp);
-    cfac = (T)exp(lncum + sc.lnpair[sc.pair] - *sc.ref);
-  }
+  const bool in = q < L.V;
+  // this launch's loads first (the chain's previous rank, g, v, the partner's
+  // rows and acc): the scales below wait on the previous launch's maxima while
+  // they are in flight
   const T Lw = (T(1) - S) / T(4);
-  float m = 0.f;
-  if (q < L.V) {
+  long long r = 0;
+  T o[6], d[6], pa[6], dpa[6];
+  T graw = T(0), vv = T(0), acc0 = T(0);
+#pragma unroll
+  for (int k = 0; k < 6; k++) o[k] = d[k] = T(0);
+  if (in) {
     int x, y, z;
     interior_coords(L, q, x, y, z);
-    long long r = sidx(L, x, y, z);
+    r = sidx(L, x, y, z);
     long long nbr[6];
     bool fl[6];
     neighbours(L, x, y, z, nbr, fl);
@@ -251,14 +280,31 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
         dz[5] = dn * dz[5] + (T(1) - dn) * d0;
       }
     }
-    T o[6], d[6];
     transfer_rows(S, Lw, zz, o);
     if constexpr (TANGENT) transfer_rows(S, Lw, dz, d);
-    const T graw = g[r];
+    graw = g[r];
+    if constexpr (TANGENT) vv = v[r];
+    pair_rows<T, TANGENT, CURVED>(L, Wp, dWp, M, x, nbr, fl, r, pa, dpa);
+    acc0 = acc[r];
+  }
+  T scale = T(1), cfac = T(1), pscale = T(1);
+  double lncum = 0;
+  if constexpr (SCALED) {
+    const double l0 = sc.lnv[sc.step - 1], lp = sc.lnpair[sc.pair], rf = *sc.ref;
+    float mp, mpp;
+    block_step_max2(sc.maxv, sc.step - 1, Wp ? sc.pmaxv : nullptr, sc.pair - 1, mp, mpp);
+    if (!(mp > 0.f)) mp = 1.f;
+    if (!(mpp > 0.f)) mpp = 1.f;
+    lncum = l0 + log((double)mp);
+    scale = (T)(1.f / mp);
+    pscale = (T)(1.f / mpp);
+    cfac = (T)exp(lncum + lp - rf);
+  }
+  float m = 0.f;
+  if (in) {
     const T gv = graw * scale;
-    const T vv = TANGENT ? v[r] : T(0);
     T gp[6], dgp[6];
-    pair_rank<T, TANGENT, CURVED>(L, Wp, dWp, graw, vv, M, x, nbr, fl, r, pscale, gp, dgp);
+    pair_scale<T, TANGENT>(!Wp, graw, vv, pscale, pa, dpa, gp, dgp);
     T a = T(0), da = T(0);
 #pragma unroll
     for (int k = 0; k < 6; k++) {
@@ -273,9 +319,9 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
       }
     }
     if constexpr (TANGENT)
-      acc[r] += da * cfac;
+      acc[r] = acc0 + da * cfac;
     else
-      acc[r] += a * cfac;
+      acc[r] = acc0 + a * cfac;
   }
   if constexpr (SCALED) {
     __shared__ float sm[kB2 / 64];

@@ -12,6 +12,7 @@ struct DotList {
 };
 struct VecList {
   const double* v[32];
+  double c[32];  // coefficients of combine(), passed by value to the kernel
   int count;
 };
 
@@ -28,27 +29,49 @@ struct DevBuf {
   ~DevBuf() {
     if (p) (void)hipFree(p);
   }
+  void reset(size_t n_) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    alloc(n_);
+  }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
 // vector operations of length n on one stream; reductions are deterministic
 struct VecOps {
+  static constexpr int kHostWords = 128;
   long long n = 0;
   int nblk = 0;
   hipStream_t stream = nullptr;
   double* partial = nullptr;
   double* dres = nullptr;
-  double* dcoef = nullptr;
-  void init(long long n, hipStream_t st);
+  double* hres = nullptr;  // pinned host words for results read back
+  void init(long long n, hipStream_t st);  // no-op when already set up for n on st
   ~VecOps();
-  void dots(const DotList& dl, double* host_out);
+  void dots(const DotList& dl, double* host_out);  // waits for the results
+  void dots_dev(const DotList& dl, double* dout);  // results stay on the device
+  void fetch(const double* dsrc, int count, double* host_out);  // one copy, then wait
   double dot(const double* a, const double* b);
-  void combine(const VecList& vl, const double* host_c, double* y);
+  void combine(const VecList& vl, double* y);  // y = sum_k vl.c[k] vl.v[k]
+  // y = v_last - sum_{k < count-1} dc[k] v_k, dc on the device (Gram-Schmidt)
+  void subtract_projections(const VecList& vl, const double* dc, double* y);
   void axpby(double a, const double* x, double b, double* y);
   void sub(const double* x, const double* y, double* z);
   void copy(const double* x, double* y);
 };
+
+// the solvers' device workspace, kept by the System between solves so that a
+// solve allocates and frees nothing once the sizes have been seen
+struct SolveWork {
+  VecOps V;
+  DevBuf<double> vecs;  // nvec vectors of length n
+  DevBuf<double> gs;    // device-side Gram-Schmidt coefficients
+  long long n = 0;
+  size_t nvec = 0;
+  double* vec(size_t k) { return vecs.p + k * (size_t)n; }
+};
+SolveWork& solve_work(System& S, long long n, size_t nvec);
 
 void diis_coeffs(const std::vector<double>& A, int n, std::vector<double>& X);
 void solve_diis(System& S, const sfx_solver_opts& o, sfx_solve_stats& st);

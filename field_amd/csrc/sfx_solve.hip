@@ -17,6 +17,7 @@ namespace sfx {
 constexpr int kDotBlock = 256;
 constexpr int kMaxDots = 16;
 constexpr int kMaxVecs = 32;  // VecList capacity
+constexpr int kGsWords = 2 * kMaxVecs + 1;  // two Gram-Schmidt passes and a norm
 
 // partial[d][block] = sum_i a_d[i] * b_d[i]
 static __global__ void __launch_bounds__(kDotBlock) k_multidot(DotList dl, long long n, double* partial) {
@@ -56,12 +57,24 @@ static __global__ void __launch_bounds__(kDotBlock) k_reduce_rows(const double* 
   }
 }
 
-// y = sum_k c_k v_k  (c in device memory)
-static __global__ void k_combine(VecList vl, const double* c, long long n, double* y) {
+// y = sum_k c_k v_k  (c passed by value)
+static __global__ void k_combine(VecList vl, long long n, double* y) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0;
-  for (int k = 0; k < vl.count; k++) s += c[k] * vl.v[k][i];
+  for (int k = 0; k < vl.count; k++) s += vl.c[k] * vl.v[k][i];
+  y[i] = s;
+}
+
+// y = v_last - sum_{k < count-1} dc[k] v_k  (dc in device memory): one
+// Gram-Schmidt pass with the projections still on the device
+static __global__ void k_subtract_projections(VecList vl, const double* __restrict__ dc, long long n, double* y) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0;
+  const int last = vl.count - 1;
+  for (int k = 0; k < last; k++) s += (-dc[k]) * vl.v[k][i];
+  s += 1.0 * vl.v[last][i];
   y[i] = s;
 }
 
@@ -81,25 +94,39 @@ static __global__ void k_copy(const double* x, double* y, long long n) {
 }
 
 void VecOps::init(long long n_, hipStream_t st) {
+  if (partial && n == n_ && stream == st) return;
+  if (partial) (void)hipFree(partial);
+  if (dres) (void)hipFree(dres);
+  partial = dres = nullptr;
   n = n_;
   stream = st;
   nblk = (int)std::min<long long>(grid1(n, kDotBlock), 1024);
   SFX_HIP(hipMalloc(&partial, sizeof(double) * kMaxDots * nblk));
   SFX_HIP(hipMalloc(&dres, sizeof(double) * kMaxDots));
-  SFX_HIP(hipMalloc(&dcoef, sizeof(double) * kMaxVecs));
+  if (!hres) SFX_HIP(hipHostMalloc(&hres, sizeof(double) * kHostWords, hipHostMallocDefault));
 }
 VecOps::~VecOps() {
   if (partial) (void)hipFree(partial);
   if (dres) (void)hipFree(dres);
-  if (dcoef) (void)hipFree(dcoef);
+  if (hres) (void)hipHostFree(hres);
 }
-void VecOps::dots(const DotList& dl, double* host_out) {
+void VecOps::dots_dev(const DotList& dl, double* dout) {
   SFX_REQUIRE(dl.count <= kMaxDots, SFX_E_INVALID, "too many dot products");
   if (dl.count <= 0) return;
   k_multidot<<<nblk, kDotBlock, 0, stream>>>(dl, n, partial);
-  k_reduce_rows<<<dl.count, kDotBlock, 0, stream>>>(partial, nblk, dres);
-  SFX_HIP(hipMemcpyAsync(host_out, dres, sizeof(double) * dl.count, hipMemcpyDeviceToHost, stream));
+  k_reduce_rows<<<dl.count, kDotBlock, 0, stream>>>(partial, nblk, dout);
+}
+void VecOps::fetch(const double* dsrc, int count, double* host_out) {
+  SFX_REQUIRE(count >= 0 && count <= kHostWords, SFX_E_INVALID, "too many words to read back");
+  if (count == 0) return;
+  SFX_HIP(hipMemcpyAsync(hres, dsrc, sizeof(double) * count, hipMemcpyDeviceToHost, stream));
   SFX_HIP(hipStreamSynchronize(stream));
+  std::memcpy(host_out, hres, sizeof(double) * count);
+}
+void VecOps::dots(const DotList& dl, double* host_out) {
+  if (dl.count <= 0) return;
+  dots_dev(dl, dres);
+  fetch(dres, dl.count, host_out);
 }
 double VecOps::dot(const double* a, const double* b) {
   DotList dl{};
@@ -110,11 +137,28 @@ double VecOps::dot(const double* a, const double* b) {
   dots(dl, &r);
   return r;
 }
-void VecOps::combine(const VecList& vl, const double* host_c, double* y) {
+void VecOps::combine(const VecList& vl, double* y) {
   SFX_REQUIRE(vl.count >= 1 && vl.count <= kMaxVecs, SFX_E_INVALID, "combination of 1..32 vectors");
-  SFX_HIP(hipMemcpyAsync(dcoef, host_c, sizeof(double) * vl.count, hipMemcpyHostToDevice, stream));
-  k_combine<<<grid1(n, 256), 256, 0, stream>>>(vl, dcoef, n, y);
+  k_combine<<<grid1(n, 256), 256, 0, stream>>>(vl, n, y);
 }
+void VecOps::subtract_projections(const VecList& vl, const double* dc, double* y) {
+  SFX_REQUIRE(vl.count >= 1 && vl.count <= kMaxVecs, SFX_E_INVALID, "combination of 1..32 vectors");
+  k_subtract_projections<<<grid1(n, 256), 256, 0, stream>>>(vl, dc, n, y);
+}
+SolveWork& solve_work(System& S, long long n, size_t nvec) {
+  if (!S.work) S.work = new SolveWork();
+  SolveWork& w = *S.work;
+  if (w.n != n || w.nvec < nvec) {  // grows only: a hipFree waits for the device
+    w.vecs.reset((size_t)n * nvec);
+    w.n = n;
+    w.nvec = nvec;
+  }
+  if (!w.gs.p) w.gs.alloc(kGsWords);
+  w.V.init(n, S.stream);
+  return w;
+}
+void solve_work_free(SolveWork* w) { delete w; }
+
 void VecOps::axpby(double a, const double* x, double b, double* y) {
   k_axpby<<<grid1(n, 256), 256, 0, stream>>>(a, x, b, y, n);
 }
@@ -171,35 +215,36 @@ void solve_diis(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
   const int m = o.diis_m;
   const double eta = o.diis_eta;
   SFX_REQUIRE(m >= 1 && m <= 15, SFX_E_INVALID, "diis_m must be in 1..15");
-  VecOps V;
-  V.init(n, S.stream);
-  DevBuf<double> x0(n), xR((size_t)m * n), x_x0((size_t)m * n);
+  SolveWork& W = solve_work(S, n, 2 * (size_t)m + 1);
+  VecOps& V = W.V;
+  double* x0 = W.vec(0);
+  auto xR = [&](int k) { return W.vec(1 + (size_t)k); };
+  auto x_x0 = [&](int k) { return W.vec(1 + (size_t)m + k); };
   double* x = S.d_u;
   double* g = S.d_f;
-  auto vec = [&](DevBuf<double>& b, int k) { return b.p + (size_t)k * n; };
   hipEvent_t e0, e1;
   SFX_HIP(hipEventCreate(&e0));
   SFX_HIP(hipEventCreate(&e1));
   SFX_HIP(hipEventRecord(e0, S.stream));
   std::vector<double> Aij(m * m, 0.0), Apij, Ci;
   int it = 0, k_diis = 1, k = 0;
-  V.copy(x, x0.p);  // remember the starting point for x_x0 = x - x0 (x0 = 0 in MPS.cpp:264)
+  V.copy(x, x0);  // remember the starting point for x_x0 = x - x0 (x0 = 0 in MPS.cpp:264)
   S.residual_internal(SFX_FP64);
   st.residual_evals++;
   V.axpby(-eta, g, 1.0, x);
-  V.sub(x, x0.p, vec(x_x0, 0));
-  V.copy(x, vec(xR, 0));
+  V.sub(x, x0, x_x0(0));
+  V.copy(x, xR(0));
   double err = std::sqrt(V.dot(g, g));
   while (err > o.tolerance && it < o.max_iter) {
     it++;
-    V.copy(x, x0.p);
+    V.copy(x, x0);
     S.residual_internal(SFX_FP64);
     st.residual_evals++;
     k = it % m;
     k_diis++;
     V.axpby(-eta, g, 1.0, x);
-    V.copy(x, vec(xR, k));
-    V.sub(x, x0.p, vec(x_x0, k));
+    V.copy(x, xR(k));
+    V.sub(x, x0, x_x0(k));
     if (k_diis > m) {
       k_diis = m;
       for (int i = 1; i < m; i++)
@@ -210,8 +255,8 @@ void solve_diis(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     for (int i = 0; i < k_diis; i++) {
       int posi = k - k_diis + 1 + i;
       if (posi < 0) posi += m;
-      dl.a[i] = vec(x_x0, posi);
-      dl.b[i] = vec(x_x0, k);
+      dl.a[i] = x_x0(posi);
+      dl.b[i] = x_x0(k);
     }
     dl.a[k_diis] = g;
     dl.b[k_diis] = g;
@@ -220,7 +265,7 @@ void solve_diis(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     for (int i = 0; i < k_diis; i++) Aij[i + m * (k_diis - 1)] = Aij[k_diis - 1 + m * i] = d[i];
     err = std::sqrt(d[k_diis]);
     if (err <= o.tolerance) {
-      V.copy(x0.p, x);  // return the point whose residual was measured
+      V.copy(x0, x);  // return the point whose residual was measured
       break;
     }
     Apij.assign(k_diis * k_diis, 0.0);
@@ -235,9 +280,10 @@ void solve_diis(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     for (int i = 0; i < k_diis; i++) {
       int posi = k - k_diis + 1 + i;
       if (posi < 0) posi += m;
-      vl.v[i] = vec(xR, posi);
+      vl.v[i] = xR(posi);
+      vl.c[i] = Ci[i];
     }
-    V.combine(vl, Ci.data(), x);
+    V.combine(vl, x);
     if (!std::isfinite(err)) throw Error{SFX_E_NONFINITE, "non-finite residual in DIIS"};
     if (o.verbose) std::fprintf(stderr, "diis it=%d |f|=%.6e\n", it, err);
   }

@@ -115,6 +115,8 @@ struct Profiler {
 struct Tangent;  // JVP / VJP buffers (sfx_adjoint.hip)
 void tangent_free(Tangent* t);
 struct Precond;
+struct SolveWork;  // solver vectors (sfx_solve.hip)
+void solve_work_free(SolveWork* w);
 
 struct System {
   Lattice lat;
@@ -141,6 +143,7 @@ struct System {
   float* d_swf32 = nullptr;   // ns * M
   double* d_phi_m = nullptr;  // nm * ns * M
   void* d_acc = nullptr;      // nm * ns * M accumulators (double or float)
+  void* d_acc2 = nullptr;     // the backward chain's, in a middle-out sweep (blocked lattices)
   double* d_phi = nullptr;    // ns * M
   double* d_rho = nullptr;    // ns * M
   double* d_phitot = nullptr; // M
@@ -152,6 +155,7 @@ struct System {
   long long* d_sites = nullptr;
   void* d_store = nullptr;    // maxstore * M propagator words
   void* d_pp = nullptr;       // 2 * M running propagator
+  void* d_pp2 = nullptr;      // 2 * M, the backward chain's in a middle-out sweep
   double* d_molscal = nullptr;  // nm * MS_COUNT
   double* d_lnf = nullptr;      // nm * (maxN+2)
   double* d_lnb = nullptr;
@@ -166,6 +170,7 @@ struct System {
   int npart = 0;
   Tangent* tan = nullptr;
   Precond* precond = nullptr;  // Newton-Krylov preconditioner, built on first use
+  SolveWork* work = nullptr;   // solver vectors, allocated on first use
   Profiler prof;
   bool phi_valid = false;
   int last_precision = SFX_FP64;
@@ -189,6 +194,14 @@ struct System {
   void sweep(int m, bool scaled);
   template <typename T>
   void sweep_blocked(int m, bool scaled);
+  // a first-order chain with a full store on a blocked lattice is swept
+  // middle-out: its forward and backward chains advance in the same launches.
+  // The base and the tangent sweeps must agree on this (their compositions
+  // share the reference log).
+  bool middle_out(const Molecule& mo) const {
+    return use_block3 && schedule == SFX_SCHEDULE_AUTO && mo.kind == 1 && !mo.plan.c && mo.N >= 2;
+  }
+  int schedule = SFX_SCHEDULE_AUTO;  // sfx_options.chain_schedule
 };
 
 // Fourier-space Newton-Krylov preconditioner (sfx_precond.hip)

@@ -62,13 +62,14 @@ static T* dalloc(size_t n) {
 }
 
 System::~System() {
-  void* ptrs[] = {d_u, d_f, d_swf, d_swf32, d_phi_m, d_acc, d_phi, d_rho, d_phitot, d_fprof, d_frozen,
-                  d_st_itvar, d_mol_uses, d_itmask, d_sites, d_store, d_pp, d_molscal, d_lnf, d_lnb,
+  void* ptrs[] = {d_u, d_f, d_swf, d_swf32, d_phi_m, d_acc, d_acc2, d_phi, d_rho, d_phitot, d_fprof, d_frozen,
+                  d_st_itvar, d_mol_uses, d_itmask, d_sites, d_store, d_pp, d_pp2, d_molscal, d_lnf, d_lnb,
                   d_maxf, d_maxb, d_partial, d_phib_state, d_cbulk, d_scalar, d_xabi, d_gmax};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   tangent_free(tan);
   precond_free(precond);
+  solve_work_free(work);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -134,6 +135,9 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
     const int sk = opt ? opt->sweep_kernel : SFX_SWEEP_AUTO;
     SFX_REQUIRE(sk >= SFX_SWEEP_AUTO && sk <= SFX_SWEEP_BLOCKED, SFX_E_INVALID, "unknown sweep_kernel");
     use_block3 = sk != SFX_SWEEP_PER_RANK && block3_ok(lat.d);
+    schedule = opt ? opt->chain_schedule : SFX_SCHEDULE_AUTO;
+    SFX_REQUIRE(schedule == SFX_SCHEDULE_AUTO || schedule == SFX_SCHEDULE_SEQUENTIAL, SFX_E_INVALID,
+                "unknown chain_schedule");
   }
   chi.assign(sd.chi, sd.chi + ns * ns);
   for (int i = 0; i < ns; i++)
@@ -274,6 +278,10 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
                 "memory budget below the checkpointed propagator store (2 sqrt(N) ranks)");
   d_store = dalloc<double>((size_t)store_slots * comps * M);  // sized for FP64 so both precisions fit
   d_pp = dalloc<double>(2 * comps * M);
+  if (use_block3) {
+    d_acc2 = dalloc<double>((size_t)nm * ns * M);
+    d_pp2 = dalloc<double>(2 * M);
+  }
   d_molscal = dalloc<double>((size_t)nm * MS_COUNT);
   d_lnf = dalloc<double>((size_t)nm * (maxN + 2));
   d_lnb = dalloc<double>((size_t)nm * (maxN + 2));
@@ -299,6 +307,7 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
   // zero everything once: halo slots of propagator arrays are never written
   zero_bytes(d_store, (size_t)store_slots * comps * M * 8, stream);
   zero_bytes(d_pp, 2 * comps * M * 8, stream);
+  if (d_pp2) zero_bytes(d_pp2, 2 * M * 8, stream);
   zero_bytes(d_u, n_itvar * M * 8, stream);
   zero_bytes(d_f, n_itvar * M * 8, stream);
   zero_bytes(d_acc, (size_t)nm * ns * M * 8, stream);
@@ -549,32 +558,128 @@ void System::sweep_blocked(int m, bool scaled) {
   auto bsc = [&](int step, int pair) { return scaled ? RankScale{maxb, lnb, lnf, scal + MS_REF, step, pair} : none; };
   auto state_of_rank = [&](int s) { return mo.seq[s - 1]; };
   auto nosplit = [](int) { return false; };
-  // one launch of ranks sa..sb of a chain from the rank in `in`
-  auto launch = [&](int sa, int sb, const T* in, int t, unsigned* maxv, double* lnv, const T* const* outs,
-                    const T* const* pairs, const int* pidx, bool connect) {
+  // the arguments of ranks sa..sb of a chain from the rank in `in`; with an
+  // accumulator `acc` each rank is composed with pairs[k], whose log scale is
+  // lnpair[pidx[k]]
+  auto args = [&](int sa, int sb, const T* in, int t, unsigned* maxv, double* lnv, const double* lnpair,
+                  const T* const* outs, const T* const* pairs, const int* pidx, T* acc_t) {
     B3Args<T> a{};
     a.in = in;
     a.g = gp(t);
     a.nr = sb - sa + 1;
     for (int k = 0; k < a.nr; k++) {
       a.out[k] = const_cast<T*>(outs[k]);
-      if (connect) {
+      if (acc_t) {
         a.pair[k] = pairs[k];
         a.pidx[k] = pidx[k];
       }
     }
-    a.acc = connect ? accp(t) : nullptr;
+    a.acc = acc_t;
     if (scaled) {
       a.maxv = maxv;
       a.lnv = lnv;
-      a.lnpair = lnf;
+      a.lnpair = lnpair;
       a.ref = scal + MS_REF;
       a.gmax = d_gmax + (size_t)t * kStepWords;
     }
     a.s0 = sa - 1;
-    const auto ev = prof.events(connect ? 1 : 0, P, a.nr);
-    launch_block3<T>(L, a, false, scaled, connect, st, ev.first, ev.second);
+    return a;
   };
+  auto go = [&](const B3Launch<T>& la, bool connect) {
+    int nr = 0;
+    for (int c = 0; c < la.nchains; c++) nr += la.a[c].nr;
+    const auto ev = prof.events(connect ? 1 : 0, P, nr);
+    launch_block3<T>(L, la, false, scaled, connect, st, ev.first, ev.second);
+  };
+  // one launch of ranks sa..sb of one chain
+  auto launch = [&](int sa, int sb, const T* in, int t, unsigned* maxv, double* lnv, const T* const* outs,
+                    const T* const* pairs, const int* pidx, bool connect) {
+    B3Launch<T> la{};
+    la.a[0] = args(sa, sb, in, t, maxv, lnv, lnf, outs, pairs, pidx, connect ? accp(t) : nullptr);
+    la.nchains = 1;
+    go(la, connect);
+  };
+  if (middle_out(mo)) {
+    // Launch i advances the forward chain through ranks 2..h (stored) and the
+    // backward chain through ranks N-1..h+1 (stored) together. Then the
+    // forward chain continues through ranks h+1..N composed with the stored
+    // backward ranks and the backward chain through ranks h..1 composed with
+    // the stored forward ranks, into a second accumulator: every rank is
+    // composed once. Slot s-1 holds rank s of whichever chain stored it.
+    const int h = N / 2;
+    T* acc2 = (T*)d_acc2;
+    auto acc2p = [&](int t) { return acc2 + ((size_t)m * ns + t) * Mn; };
+    T* qq[2] = {(T*)d_pp2, (T*)d_pp2 + Mn};
+    for (int t = 0; t < ns; t++)
+      if (mo.uses[t]) zero_bytes(acc2p(t), (size_t)Mn * sizeof(T), st);
+    auto rank_of = [&](int k) { return N - k + 1; };  // backward step k holds rank N-k+1
+    auto bstate = [&](int k) { return mo.seq[rank_of(k) - 1]; };
+    launch_init_rank<T>(L, gp(mo.seq[0]), slot(0), fsc(1, 0), st);
+    launch_init_rank<T>(L, gp(mo.seq[N - 1]), slot(N - 1), bsc(1, N), st);
+    const auto F1 = chain_launches(2, h, 2, state_of_rank);
+    const auto B1 = chain_launches(2, N - h, 2, bstate);
+    for (size_t i = 0; i < std::max(F1.size(), B1.size()); i++) {
+      B3Launch<T> la{};
+      if (i < F1.size()) {
+        const int sa = F1[i].first, sb = F1[i].second;
+        const T* outs[kB3Ranks] = {};
+        for (int s = sa; s <= sb; s++) outs[s - sa] = slot(s - 1);
+        la.a[la.nchains++] = args(sa, sb, slot(sa - 2), mo.seq[sa - 1], maxf, lnf, nullptr, outs, nullptr, nullptr,
+                                  nullptr);
+      }
+      if (i < B1.size()) {
+        const int ka = B1[i].first, kb = B1[i].second;
+        const T* outs[kB3Ranks] = {};
+        for (int k = ka; k <= kb; k++) outs[k - ka] = slot(rank_of(k) - 1);
+        la.a[la.nchains++] = args(ka, kb, slot(rank_of(ka - 1) - 1), bstate(ka), maxb, lnb, nullptr, outs, nullptr,
+                                  nullptr, nullptr);
+      }
+      go(la, false);
+    }
+    launch_set_ref_pair(scal, scaled ? lnf : nullptr, h, lnb, N - h, st);
+    const auto F2 = chain_launches(h + 1, N, h + 1, state_of_rank);
+    const auto B2 = chain_launches(N - h + 1, N, N - h + 1, bstate);
+    int fc = 0, bc = 0;
+    for (size_t i = 0; i < std::max(F2.size(), B2.size()); i++) {
+      B3Launch<T> la{};
+      if (i < F2.size()) {
+        const int sa = F2[i].first, sb = F2[i].second;
+        const T* outs[kB3Ranks] = {};
+        const T* pairs[kB3Ranks] = {};
+        int pidx[kB3Ranks] = {};
+        for (int s = sa; s <= sb; s++) {
+          pairs[s - sa] = slot(s - 1);  // backward rank s
+          pidx[s - sa] = N - s + 1;     // its backward step
+        }
+        outs[sb - sa] = pp[fc ^ 1];
+        const int t = mo.seq[sa - 1];
+        la.a[la.nchains++] = args(sa, sb, i == 0 ? slot(h - 1) : pp[fc], t, maxf, lnf, lnb, outs, pairs, pidx, accp(t));
+        fc ^= 1;
+      }
+      if (i < B2.size()) {
+        const int ka = B2[i].first, kb = B2[i].second;
+        const T* outs[kB3Ranks] = {};
+        const T* pairs[kB3Ranks] = {};
+        int pidx[kB3Ranks] = {};
+        for (int k = ka; k <= kb; k++) {
+          pairs[k - ka] = slot(rank_of(k) - 1);  // forward rank
+          pidx[k - ka] = rank_of(k);
+        }
+        outs[kb - ka] = qq[bc ^ 1];
+        const int t = bstate(ka);
+        la.a[la.nchains++] = args(ka, kb, i == 0 ? slot(h) : qq[bc], t, maxb, lnb, lnf, outs, pairs, pidx, acc2p(t));
+        bc ^= 1;
+      }
+      go(la, true);
+    }
+    int nb = launch_wsum<T>(L, pp[fc], d_partial, st);  // forward rank N
+    launch_mol_finalize(mo.params(m), d_partial, nb, scal, scaled ? lnf + N : nullptr, d_molscal, nm, st);
+    for (int t = 0; t < ns; t++)
+      if (mo.uses[t])
+        launch_mol_phi<T>(L, accp(t), d_swf + (size_t)t * Mn, scal, 0, d_phi_m + ((size_t)m * ns + t) * Mn, st,
+                          acc2p(t));
+    return;
+  }
   // forward chain, ranks 2..plan.n; every rank kept (full store) or the
   // checkpoints plus the last rank of each launch, alternating between two
   // segment slots so a launch never overwrites its own input

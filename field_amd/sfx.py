@@ -128,9 +128,10 @@ class System:
     """One SCF system on the GPU (sfx_system_create)."""
 
     _SWEEP = {"auto": _abi.SWEEP_AUTO, "per_rank": _abi.SWEEP_PER_RANK, "blocked": _abi.SWEEP_BLOCKED}
+    _SCHEDULE = {"auto": _abi.SCHEDULE_AUTO, "sequential": _abi.SCHEDULE_SEQUENTIAL}
 
     def __init__(self, lat: LatticeSpec, sysd: SystemSpec, precision: str = "fp64",
-                 memory_budget: int = 0, sweep_kernel: str = "auto"):
+                 memory_budget: int = 0, sweep_kernel: str = "auto", chain_schedule: str = "auto"):
         self.lat, self.sys = lat, sysd
         self._ld = lat.desc()
         self._sd, self._keep = sysd.desc(lat)
@@ -138,6 +139,7 @@ class System:
         opt.precision = _PREC[precision]
         opt.memory_budget = memory_budget
         opt.sweep_kernel = self._SWEEP[sweep_kernel]
+        opt.chain_schedule = self._SCHEDULE[chain_schedule]
         self.precision = opt.precision
         h = C.c_void_p()
         check(lib().sfx_system_create(C.byref(self._ld), C.byref(self._sd), C.byref(opt), C.byref(h)))

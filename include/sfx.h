@@ -121,9 +121,19 @@ typedef struct sfx_options {
                              SFX_SWEEP_PER_RANK (one launch per rank) or
                              SFX_SWEEP_BLOCKED (up to four ranks per launch through
                              LDS, where every side exceeds four layers) */
+  int chain_schedule;     /* SFX_SCHEDULE_AUTO: a first-order chain without mirror
+                             symmetry, fully stored, on a blocked lattice advances its
+                             forward and backward chains in the same launches (each
+                             stores half and composes the other half); elsewhere, and
+                             with SFX_SCHEDULE_SEQUENTIAL, the forward chain is stored
+                             and the backward chain composes with it
+                             (SF_Copol1stO.cpp:1186-1281). The two schedules agree to
+                             rounding; a checkpointed store repeats the sequential one
+                             bit for bit. No counterpart in the reference. */
 } sfx_options;
 
 enum { SFX_SWEEP_AUTO = 0, SFX_SWEEP_PER_RANK = 1, SFX_SWEEP_BLOCKED = 2 };
+enum { SFX_SCHEDULE_AUTO = 0, SFX_SCHEDULE_SEQUENTIAL = 1 };
 
 typedef struct sfx_system sfx_system;
 

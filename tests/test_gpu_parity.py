@@ -119,8 +119,10 @@ def test_negative_solvent_bulk_fraction(sfx, orc):
 def test_checkpointed_sweeps_match_full_store(built, name, sfx):
     """A memory budget below the full propagator store switches to sqrt(N)
     checkpoints with recomputation; residuals and JVPs are bit-identical to the
-    full store in both precisions."""
-    lat, sysd, G, O = built[name]
+    full store swept in the same order (forward, then backward) in both
+    precisions."""
+    lat, sysd, _, O = built[name]
+    G = sfx.System(lat, sysd, chain_schedule="sequential")
     comps = 6 if lat.order == 2 else 1
     full = cases.store_ranks(lat, sysd) * comps * G.M * 8
     C = sfx.System(lat, sysd, memory_budget=full - 1)
@@ -131,6 +133,27 @@ def test_checkpointed_sweeps_match_full_store(built, name, sfx):
         np.testing.assert_array_equal(C.jvp(x, v, precision=prec), G.jvp(x, v, precision=prec))
     with pytest.raises(sfx.SFError):
         sfx.System(lat, sysd, memory_budget=2 * comps * G.M * 8)
+
+
+@pytest.mark.parametrize("name", ["3d_sc_melt_n24", "3d_sc_mirrors_blocked"])
+def test_middle_out_matches_sequential(built, name, sfx):
+    """The middle-out schedule (forward and backward chains in the same
+    launches, each composing half the ranks) against the sequential one:
+    residuals and JVPs agree to rounding in both precisions, and the FP64
+    residual matches the oracle."""
+    lat, sysd, G, O = built[name]
+    Q = sfx.System(lat, sysd, chain_schedule="sequential")
+    x = cases.start(lat, sysd, seed=23, sigma=0.1)
+    v = np.random.default_rng(24).normal(0, 1, G.nvar)
+    f = G.residual(x, precision="fp64")
+    np.testing.assert_allclose(f, O.residual(x), rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(f, Q.residual(x, precision="fp64"), rtol=1e-12, atol=1e-13)
+    j = G.jvp(x, v, precision="fp64")
+    np.testing.assert_allclose(j, Q.jvp(x, v, precision="fp64"), rtol=1e-11, atol=1e-12 * np.abs(j).max())
+    f32, q32 = G.residual(x, precision="fp32"), Q.residual(x, precision="fp32")
+    assert np.abs(f32 - q32).max() < 1e-5 * max(1.0, np.abs(f).max())
+    j32, k32 = G.jvp(x, v, precision="fp32"), Q.jvp(x, v, precision="fp32")
+    assert np.abs(j32 - k32).max() < 1e-4 * np.abs(j).max()
 
 
 @pytest.mark.parametrize("name", cases.NAMES)
