@@ -181,6 +181,184 @@ __global__ void __launch_bounds__(kPB) k_apply_block(const cf* __restrict__ in, 
   for (int i = 0; i < nv; i++) out[(long long)i * V + q] = cf{(float)re[i], (float)im[i]};
 }
 
+// ---------------------------------------------------------------------------
+// Fused path for power-of-two sides with a y-z plane of at most kPlane
+// points: three passes instead of nine.
+//   k_plane_fwd: pack one (field, x) plane of the residual and FFT it along z,
+//                then along y, in LDS;
+//   k_xline:     per group of (y, z) lines, all fields: FFT along x, the block
+//                solve per wave vector, inverse FFT along x;
+//   k_plane_inv: inverse FFT of one plane along y, then z, real part / V into
+//                the [itvar][M] layout (halos and off-site entries 0).
+// The butterflies are those of k_fft (radix-2 Stockham, natural order).
+constexpr int kPlane = 4096;
+constexpr int kPlaneThreads = 1024;
+constexpr int kXTw = 512;   // longest x side of the fused path (twiddles in LDS)
+constexpr int kXPts = 2048;  // points (all fields) per k_xline workgroup
+
+// one radix-2 Stockham stage over `lines` lines of n points: element j of line
+// l sits at l * ls + j * js in src and dst
+__device__ __forceinline__ void stockham_stage(const cf* src, cf* dst, int lines, int n, int Ns, int ls, int js,
+                                               const cf* tw, int nthreads) {
+  const int half = n >> 1;
+  for (int b = threadIdx.x; b < lines * half; b += nthreads) {
+    // lines fastest when they are the contiguous direction (js > 1)
+    const int l = js > 1 ? b % lines : b / half;
+    const int jb = js > 1 ? b / lines : b % half;
+    const cf v0 = src[l * ls + jb * js], v1 = src[l * ls + (jb + half) * js];
+    const int k = jb & (Ns - 1);
+    const cf w = tw[k * (n / (2 * Ns))];
+    const float tx = v1.x * w.x - v1.y * w.y, ty = v1.x * w.y + v1.y * w.x;
+    const int d = ((jb - k) << 1) + k;
+    dst[l * ls + d * js] = cf{v0.x + tx, v0.y + ty};
+    dst[l * ls + (d + Ns) * js] = cf{v0.x - tx, v0.y - ty};
+  }
+}
+
+// FFT of every line of an LDS array in place (result back in buf[0] or
+// buf[1]; returns which)
+__device__ __forceinline__ int lds_fft(cf* buf0, cf* buf1, int cur, int lines, int n, int ls, int js,
+                                       const cf* tw, int nthreads) {
+  cf* b[2] = {buf0, buf1};
+  for (int Ns = 1; Ns < n; Ns <<= 1) {
+    __syncthreads();
+    stockham_stage(b[cur], b[cur ^ 1], lines, n, Ns, ls, js, tw, nthreads);
+    cur ^= 1;
+  }
+  __syncthreads();
+  return cur;
+}
+
+// the twiddles of the y and z transforms of a plane (ny + nz <= kPlane / 2 + 2
+// entries) copied into LDS once per workgroup
+__device__ __forceinline__ void load_plane_twiddles(cf* twl, const cf* __restrict__ twy, int ny,
+                                                    const cf* __restrict__ twz, int nz) {
+  for (int e = threadIdx.x; e < ny + nz; e += blockDim.x) twl[e] = e < ny ? twy[e] : twz[e - ny];
+}
+
+__global__ void __launch_bounds__(kPlaneThreads) k_plane_fwd(DevLattice L, const double* __restrict__ r, int nv,
+                                                             cf* __restrict__ out, const cf* __restrict__ twy,
+                                                             const cf* __restrict__ twz) {
+  __shared__ cf buf[2][kPlane + kPlane / 2];  // rows of nz + 1
+  __shared__ cf twl[kPlane / 2 + 2];
+  const int ny = L.ny, nz = L.nz, P = nz + 1;
+  const int f = blockIdx.x / L.nx, x = blockIdx.x % L.nx;
+  load_plane_twiddles(twl, twy, ny, twz, nz);
+  for (int e = threadIdx.x; e < ny * nz; e += kPlaneThreads) {
+    const int y = e / nz, z = e % nz;
+    buf[0][y * P + z] = cf{(float)r[(long long)f * L.M + sidx(L, x + 1, y + 1, z + 1)], 0.f};
+  }
+  int cur = lds_fft(buf[0], buf[1], 0, ny, nz, P, 1, twl + ny, kPlaneThreads);  // rows: along z
+  cur = lds_fft(buf[0], buf[1], cur, nz, ny, 1, P, twl, kPlaneThreads);         // columns: along y
+  cf* o = out + (long long)f * L.V + (long long)x * ny * nz;
+  for (int e = threadIdx.x; e < ny * nz; e += kPlaneThreads) o[e] = buf[cur][(e / nz) * P + e % nz];
+}
+
+template <int NV>
+__global__ void __launch_bounds__(kPB) k_xline(const cf* __restrict__ in, cf* __restrict__ out,
+                                               const float* __restrict__ j0, int nx, long long nyz, int LB,
+                                               const cf* __restrict__ twf, const cf* __restrict__ twi, float sigma) {
+  __shared__ cf buf[2][kXPts + kXPts / 2];  // NV * LB lines of nx + 1
+  __shared__ cf twl[2][kXTw];                // forward and inverse twiddles
+  const long long V = nx * nyz;
+  const int P = nx + 1, lines = NV * LB;
+  const long long c0 = (long long)blockIdx.x * LB;  // first (y, z) line
+  for (int e = threadIdx.x; e < nx; e += kPB) {
+    twl[0][e] = twf[e];
+    twl[1][e] = twi[e];
+  }
+  // line (f, l) at (f * LB + l) * P; loads coalesce along l (consecutive y-z)
+  for (int e = threadIdx.x; e < lines * nx; e += kPB) {
+    const int l = e % LB, j = (e / LB) % nx, f = e / (LB * nx);
+    const long long c = c0 + l;
+    buf[0][(f * LB + l) * P + j] = c < nyz ? in[(long long)f * V + j * nyz + c] : cf{0.f, 0.f};
+  }
+  int cur = lds_fft(buf[0], buf[1], 0, lines, nx, P, 1, twl[0], kPB);
+  // (sigma I + J0(q))^+ per wave vector, as k_apply_block
+  for (int e = threadIdx.x; e < LB * nx; e += kPB) {
+    const int l = e % LB, j = e / LB;
+    const long long c = c0 + l;
+    if (c >= nyz) continue;
+    const long long q = j * nyz + c;
+    double A[NV][NV], B[NV][NV], re[NV], im[NV];
+    double amax = 0;
+    for (int i = 0; i < NV; i++)
+      for (int k = 0; k < NV; k++) {
+        A[i][k] = (double)j0[((long long)i * NV + k) * V + q] + (i == k ? (double)sigma : 0.0);
+        amax = fmax(amax, fabs(A[i][k]));
+      }
+    const double mu2 = (1e-6 * amax) * (1e-6 * amax) + 1e-300;
+    for (int i = 0; i < NV; i++) {
+      double sr = 0, si = 0;
+      for (int k = 0; k < NV; k++) {
+        const cf v = buf[cur][(k * LB + l) * P + j];
+        sr += A[k][i] * v.x;
+        si += A[k][i] * v.y;
+      }
+      re[i] = sr;
+      im[i] = si;
+      for (int m = 0; m < NV; m++) {
+        double sacc = 0;
+        for (int k = 0; k < NV; k++) sacc += A[k][i] * A[k][m];
+        B[i][m] = sacc + (i == m ? mu2 : 0.0);
+      }
+    }
+    for (int cc = 0; cc < NV; cc++) {
+      const double inv = 1.0 / B[cc][cc];
+      for (int m = 0; m < NV; m++) B[cc][m] *= inv;
+      re[cc] *= inv;
+      im[cc] *= inv;
+      for (int rr = 0; rr < NV; rr++) {
+        if (rr == cc) continue;
+        const double fct = B[rr][cc];
+        if (fct == 0.0) continue;
+        for (int m = 0; m < NV; m++) B[rr][m] -= fct * B[cc][m];
+        re[rr] -= fct * re[cc];
+        im[rr] -= fct * im[cc];
+      }
+    }
+    // each thread reads and writes only its own (l, j) across the fields
+    for (int i = 0; i < NV; i++) buf[cur][(i * LB + l) * P + j] = cf{(float)re[i], (float)im[i]};
+  }
+  cur = lds_fft(buf[0], buf[1], cur, lines, nx, P, 1, twl[1], kPB);
+  for (int e = threadIdx.x; e < lines * nx; e += kPB) {
+    const int l = e % LB, j = (e / LB) % nx, f = e / (LB * nx);
+    const long long c = c0 + l;
+    if (c < nyz) out[(long long)f * V + j * nyz + c] = buf[cur][(f * LB + l) * P + j];
+  }
+}
+
+__global__ void __launch_bounds__(kPlaneThreads) k_plane_inv(DevLattice L, const cf* __restrict__ in, int nv,
+                                                             const unsigned char* __restrict__ itmask,
+                                                             double* __restrict__ out, const cf* __restrict__ twy,
+                                                             const cf* __restrict__ twz) {
+  __shared__ cf buf[2][kPlane + kPlane / 2];
+  __shared__ cf twl[kPlane / 2 + 2];
+  const int ny = L.ny, nz = L.nz, P = nz + 1;
+  const int f = blockIdx.x / L.nx, x = blockIdx.x % L.nx;
+  load_plane_twiddles(twl, twy, ny, twz, nz);
+  const cf* src = in + (long long)f * L.V + (long long)x * ny * nz;
+  for (int e = threadIdx.x; e < ny * nz; e += kPlaneThreads) buf[0][(e / nz) * P + e % nz] = src[e];
+  int cur = lds_fft(buf[0], buf[1], 0, nz, ny, 1, P, twl, kPlaneThreads);  // columns: along y
+  cur = lds_fft(buf[0], buf[1], cur, ny, nz, P, 1, twl + ny, kPlaneThreads);  // rows: along z
+  // the whole M-layout plane x + 1 (halo rows and columns 0); the first x
+  // also clears the two halo planes
+  double* o = out + (long long)f * L.M;
+  const long long plane = (long long)L.MY * L.MZ;
+  for (long long e = threadIdx.x; e < plane; e += kPlaneThreads) {
+    const int y = (int)(e / L.MZ), z = (int)(e % L.MZ);
+    const long long s = (long long)(x + 1) * plane + e;
+    double v = 0.0;
+    if (y >= 1 && y <= ny && z >= 1 && z <= nz && itmask[s])
+      v = (double)buf[cur][(y - 1) * P + (z - 1)].x / (double)L.V;
+    o[s] = v;
+    if (x == 0) {
+      o[e] = 0.0;
+      o[(long long)(L.nx + 1) * plane + e] = 0.0;
+    }
+  }
+}
+
 // real part / V -> [itvar][M] interior (halos and off-site entries 0)
 __global__ void k_unpack(DevLattice L, const cf* __restrict__ in, int nv, const unsigned char* __restrict__ itmask,
                          double* __restrict__ out) {
@@ -346,8 +524,43 @@ struct Precond {
     result = b;
   }
 
+  // the three-pass path: power-of-two sides, a y-z plane within kPlane
+  // points and every field of a x-line group within kXPts points
+  bool fused() const {
+    for (int a = 0; a < 3; a++)
+      if (!fft_len(n[a])) return false;
+    return (long long)n[1] * n[2] <= kPlane && nv * n[0] <= kXPts && n[0] <= kXTw;
+  }
+
+  template <int NV>
+  void xline(const System& S, double sigma) {
+    const int LB = kXPts / (NV * n[0]);
+    const long long nyz = (long long)n[1] * n[2];
+    const unsigned g = (unsigned)((nyz + LB - 1) / LB);
+    k_xline<NV><<<g, kPB, 0, S.stream>>>(b1.p, b0.p, j0.p, n[0], nyz, LB, tw.p + twoff[0][0], tw.p + twoff[0][1],
+                                         (float)sigma);
+  }
+
   void apply(const System& S, const double* r, double* out, double sigma) {
     const DevLattice& L = S.lat.d;
+    if (fused()) {
+      const unsigned planes = (unsigned)(n[0] * nv);
+      k_plane_fwd<<<planes, kPlaneThreads, 0, S.stream>>>(L, r, nv, b1.p, tw.p + twoff[1][0], tw.p + twoff[2][0]);
+      switch (nv) {
+        case 1: xline<1>(S, sigma); break;
+        case 2: xline<2>(S, sigma); break;
+        case 3: xline<3>(S, sigma); break;
+        case 4: xline<4>(S, sigma); break;
+        case 5: xline<5>(S, sigma); break;
+        case 6: xline<6>(S, sigma); break;
+        case 7: xline<7>(S, sigma); break;
+        default: xline<8>(S, sigma); break;
+      }
+      k_plane_inv<<<planes, kPlaneThreads, 0, S.stream>>>(L, b0.p, nv, S.d_itmask, out, tw.p + twoff[1][1],
+                                                          tw.p + twoff[2][1]);
+      SFX_HIP(hipGetLastError());
+      return;
+    }
     k_pack<<<grid1(V * nv, kPB), kPB, 0, S.stream>>>(L, r, nv, b0.p);
     cf* res = nullptr;
     dft3(S, b0.p, b1.p, 0, res);  // result in b1

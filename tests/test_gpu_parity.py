@@ -321,3 +321,31 @@ def test_per_rank_and_blocked_sweeps_agree(sfx, orc, name):
     assert np.abs(f32p - f32b).max() < 2e-5 * max(1.0, np.abs(fo).max())
     jp, jb = P.jvp(x, v, precision="fp32"), B.jvp(x, v, precision="fp32")
     assert np.abs(jp - jb).max() < 1e-4 * np.abs(jb).max()
+
+
+@pytest.mark.parametrize("n, n_a, n_b, chi", [(16, 3, 4, 0.5), (64, 12, 28, 0.38)])
+def test_fourier_preconditioner_matches_numpy(sfx, n, n_a, n_b, chi):
+    """Power-of-two sides (the three-pass path: planes FFT'd in LDS, x-lines
+    FFT'd with the block solve fused) against numpy: FFT of the residual,
+    (A^T A + mu^2 I)^-1 A^T r per wave vector with A = J0(q) the lattice-RPA
+    Jacobian (pinned against the oracle's finite differences in
+    test_spinodal.py) and mu = 1e-6 max|A|, inverse FFT. FP32 transforms."""
+    from test_spinodal import rpa_jacobian
+    lat, sysd, _ = configs.melt_3d(n=n, n_a=n_a, n_b=n_b, chi=chi)
+    G = sfx.System(lat, sysd)
+    mask = lat.interior_mask().reshape(-1)
+    r = np.random.default_rng(9).normal(0, 1, (2, G.M)) * mask
+    w = G.precondition(r.reshape(-1)).reshape(2, G.M)
+    inner = np.flatnonzero(mask)
+    R = np.fft.fftn(r[:, inner].reshape(2, n, n, n), axes=(1, 2, 3))
+    A = np.moveaxis(rpa_jacobian(n, n_a, n_b, chi), (0, 1), (-2, -1))  # (n, n, n, 2, 2)
+    mu2 = (1e-6 * np.abs(A).max(axis=(-2, -1))) ** 2
+    At = np.swapaxes(A, -1, -2)
+    B = At @ A + mu2[..., None, None] * np.eye(2)
+    rhs = (At @ np.moveaxis(R, 0, -1)[..., None])[..., 0]
+    Wq = np.linalg.solve(B, rhs[..., None])[..., 0]
+    wn = np.fft.ifftn(np.moveaxis(Wq, -1, 0), axes=(1, 2, 3)).real.reshape(2, -1)
+    err = np.abs(w[:, inner] - wn).max() / np.abs(wn).max()
+    print(f"{n}^3 preconditioner vs numpy: {err:.2e}")
+    assert err < 1e-4
+    assert np.all(w[:, ~mask] == 0)
