@@ -94,3 +94,24 @@ def test_semiflexible_full_size_properties(sfx):
     assert rel < 1e-5
     j32 = G.jvp(xd, v, precision="fp32")
     assert ((j32 - jv).abs().max() / jv.abs().max()).item() < 1e-3
+
+
+def test_semiflexible_solve_is_homogeneous(sfx):
+    """The 128^3 semiflexible system solved by Newton-Krylov with sqrt(N)
+    checkpoints forced by a memory budget, to 1e-7 in the FP64 residual.
+    chi = 0.6 lies below the spinodal (1/(N phi) + 1/(1 - phi) = 2 chi gives
+    chi_s = 0.72 at phi = 0.3, N = 200), so the solution is the homogeneous
+    phi_A = 0.3 at every site: a closed form at full size."""
+    import torch
+    lat, sysd, x0 = configs.semiflexible_3d()
+    full = 200 * 6 * lat.M * 8
+    G = sfx.System(lat, sysd, memory_budget=full // 4)
+    xs, st = G.solve(torch.tensor(x0, device="cuda"), method="nk", tolerance=1e-7, max_iter=60, gmres_rtol=0.3,
+                     jvp_precision="fp32", ptc_dt0=3.0)
+    print("semiflexible 128^3 solve:", st)
+    assert st["converged"]
+    G.residual(xs)
+    inner = lat.interior_mask().reshape(-1)
+    phi_a = G.phi(0)[inner]
+    assert np.abs(phi_a - 0.3).max() < 1e-6
+    assert np.abs(G.phi(1)[inner] - 0.7).max() < 1e-6
