@@ -167,6 +167,20 @@ int sfx_free_energy(sfx_system* s, double* F) {
   });
 }
 
+int sfx_grand_potential(sfx_system* s, double* omega) {
+  return guarded([&] {
+    SFX_REQUIRE(s && omega, SFX_E_INVALID, "NULL argument");
+    *omega = s->s.grand_potential();
+  });
+}
+
+int sfx_chemical_potential(sfx_system* s, int molecule, double* mu) {
+  return guarded([&] {
+    SFX_REQUIRE(s && mu, SFX_E_INVALID, "NULL argument");
+    *mu = s->s.chemical_potential(molecule);
+  });
+}
+
 int sfx_set_chi(sfx_system* s, int a, int b, double value) {
   return guarded([&] {
     SFX_REQUIRE(s, SFX_E_INVALID, "NULL argument");

@@ -121,6 +121,9 @@ void launch_residual(const DevLattice& L, const SysParams& sp, const unsigned ch
 int launch_free_energy(const DevLattice& L, const SysParams& sp, const double* phi, const double* phi_m,
                        const double* rho, const double* swf, const double* u, const double* molscal,
                        double* partial, hipStream_t st);
+int launch_grand_potential(const DevLattice& L, const SysParams& sp, const double* phi, const double* phi_m,
+                           const double* rho, const double* swf, const double* molscal, const double* phib_state,
+                           const double* cbulk, double* partial, hipStream_t st);
 void launch_sum_partials(const double* partial, int n, double* out, hipStream_t st);
 
 }  // namespace sfx

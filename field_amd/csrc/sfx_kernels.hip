@@ -435,6 +435,71 @@ __global__ void k_free_energy(DevLattice L, SysParams sp, const double* __restri
   if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
+// Grand potential (SF form): the profile of SF_MolList1stO.cpp:1001-1100
+// without charges, grafted chains or external potentials, weighted as the free
+// energy (SubtractBoundaries, MultiplyWithLatticeSites), summed per block.
+//   omega = -sum_mol [phi_mol > 0] (phi_mol - phi_b,mol) / N
+//           - sum_t phi_t u'_t,  u'_t = -ln g_t - ChemInt_t + ChemIntBulk_t (g_t > 0)
+//           - 1/2 sum_tj chi_tj phi_t <phi_j> + 1/2 sum_tj [phi_t > 0] chi_tj phi_b,t phi_b,j
+// (states t, j not frozen in the last two sums).
+__global__ void k_grand_potential(DevLattice L, SysParams sp, const double* __restrict__ phi,
+                                  const double* __restrict__ phi_m, const double* __restrict__ rho,
+                                  const double* __restrict__ swf, const double* __restrict__ molscal,
+                                  const double* __restrict__ phib_state, const double* __restrict__ cbulk,
+                                  double* __restrict__ partial) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0;
+  if (i < L.M) {
+    int x, y, z;
+    site_coords(L, i, x, y, z);
+    const double w = fe_weight(L, x, y, z);
+    if (w != 0) {
+      const int ns = sp.ns;
+      int sx = remap(x, L.nx, L.b1[0], L.b2[0]);
+      int sy = L.grad >= 2 ? remap(y, L.ny, L.b1[1], L.b2[1]) : y;
+      int sz = L.grad >= 3 ? remap(z, L.nz, L.b1[2], L.b2[2]) : z;
+      const long long s = sidx(L, sx, sy, sz);
+      const bool liquid_site = is_interior(L, sx, sy, sz);
+      double om = 0;
+      for (int m = 0; m < sp.nm; m++) {
+        double pm = 0;
+        if (liquid_site)
+          for (int t = 0; t < ns; t++)
+            if (sp.mol_frac[m * SFX_MAXS + t] > 0) pm += phi_m[((long long)m * ns + t) * L.M + s];
+        if (pm > 0) om -= (pm - molscal[m * MS_COUNT + MS_PHIB]) / sp.mol_N[m];
+      }
+      for (int t = 0; t < ns; t++) {
+        if (sp.frozen[t]) continue;
+        const double G = swf[(long long)t * L.M + s];
+        double up = 0;
+        if (G > 0) {
+          double ci = 0;
+          for (int j = 0; j < ns; j++) {
+            const double c = sp.chi[t * SFX_MAXS + j];
+            if (c != 0) ci += c * rho[(long long)j * L.M + i];
+          }
+          up = -log(G) - ci + cbulk[t];
+        }
+        om -= up * phi[(long long)t * L.M + i];
+      }
+      for (int t = 0; t < ns; t++) {
+        if (sp.frozen[t]) continue;
+        const double pt = phi[(long long)t * L.M + i];
+        for (int j = 0; j < ns; j++) {
+          if (sp.frozen[j]) continue;
+          const double c = sp.chi[t * SFX_MAXS + j];
+          if (c == 0) continue;
+          om -= 0.5 * c * pt * rho[(long long)j * L.M + i];
+          if (pt > 0) om += 0.5 * c * phib_state[t] * phib_state[j];
+        }
+      }
+      v = om * w;
+    }
+  }
+  v = block_sum(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
 __global__ void k_sum_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double s = 0;
@@ -537,6 +602,14 @@ int launch_free_energy(const DevLattice& L, const SysParams& sp, const double* p
                        double* partial, hipStream_t st) {
   unsigned nb = grid1(L.M, kBlock);
   k_free_energy<<<nb, kBlock, 0, st>>>(L, sp, phi, phi_m, rho, swf, u, molscal, partial);
+  return (int)nb;
+}
+
+int launch_grand_potential(const DevLattice& L, const SysParams& sp, const double* phi, const double* phi_m,
+                           const double* rho, const double* swf, const double* molscal, const double* phib_state,
+                           const double* cbulk, double* partial, hipStream_t st) {
+  const unsigned nb = grid1(L.M, kBlock);
+  k_grand_potential<<<nb, kBlock, 0, st>>>(L, sp, phi, phi_m, rho, swf, molscal, phib_state, cbulk, partial);
   return (int)nb;
 }
 

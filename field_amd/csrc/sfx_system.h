@@ -188,6 +188,8 @@ struct System {
   void compute_phi(int precision);  // from d_u
   void residual_internal(int precision);  // d_u -> d_f
   double free_energy();
+  double grand_potential();
+  double chemical_potential(int m);
   void sync();
 
   template <typename T, typename A>

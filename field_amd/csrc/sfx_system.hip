@@ -832,6 +832,43 @@ double System::free_energy() {
   return F;
 }
 
+// SF_MolList1stO.cpp:637-647 over the profile of :1001-1100 (k_grand_potential)
+double System::grand_potential() {
+  SFX_REQUIRE(phi_valid, SFX_E_INVALID, "grand potential needs a preceding compute_phi or residual");
+  SFX_REQUIRE(form != SFX_FORM_MELT, SFX_E_UNSUPPORTED, "the melt form has no grand potential");
+  int nb = launch_grand_potential(lat.d, sp, d_phi, d_phi_m, d_rho, d_swf, d_molscal, d_phib_state, d_cbulk,
+                                  d_partial, stream);
+  launch_sum_partials(d_partial, nb, d_scalar, stream);
+  double G = 0;
+  SFX_HIP(hipMemcpyAsync(&G, d_scalar, sizeof(double), hipMemcpyDeviceToHost, stream));
+  SFX_HIP(hipStreamSynchronize(stream));
+  return G;
+}
+
+// SF_MolList1stO.cpp:389-452 for first-generation freedoms with one state per
+// segment and no reactions: ln phi_b + 1 - N sum_m' phi_b,m'/N_m'
+// + N (sum_t phi_ref,t ChemIntBulk_t - chemIntRef) - N/2 sum_t phi_b,t ChemIntBulk_t
+double System::chemical_potential(int m) {
+  SFX_REQUIRE(m >= 0 && m < nm, SFX_E_INVALID, "molecule index out of range");
+  SFX_REQUIRE(phi_valid, SFX_E_INVALID, "chemical potential needs a preceding compute_phi or residual");
+  SFX_REQUIRE(form != SFX_FORM_MELT, SFX_E_UNSUPPORTED, "the melt form has no chemical potentials");
+  std::vector<double> ms((size_t)nm * MS_COUNT), pbs(ns), cb(ns);
+  SFX_HIP(hipMemcpyAsync(ms.data(), d_molscal, ms.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+  SFX_HIP(hipMemcpyAsync(pbs.data(), d_phib_state, ns * sizeof(double), hipMemcpyDeviceToHost, stream));
+  SFX_HIP(hipMemcpyAsync(cb.data(), d_cbulk, ns * sizeof(double), hipMemcpyDeviceToHost, stream));
+  SFX_HIP(hipStreamSynchronize(stream));
+  const double N = mols[m].N;
+  double mu = std::log(ms[(size_t)m * MS_COUNT + MS_PHIB]) + 1.0;
+  for (int k = 0; k < nm; k++) mu -= N * (ms[(size_t)k * MS_COUNT + MS_PHIB] / mols[k].N);
+  double d = 0;
+  for (int t = 0; t < ns; t++) d += sp.mol_frac[m * SFX_MAXS + t] * cb[t];
+  mu += N * (d - mols[m].chem_int_ref);
+  d = 0;
+  for (int t = 0; t < ns; t++) d += pbs[t] * cb[t];
+  mu -= 0.5 * N * d;
+  return mu;
+}
+
 template void System::sweep<double, double>(int, bool);
 template void System::sweep<float, float>(int, bool);
 template void System::sweep_blocked<double>(int, bool);

@@ -37,6 +37,8 @@ _SIGS = {
     "sfx_get_lngn": (C.c_int, [C.c_void_p, C.c_int, PD]),
     "sfx_get_phibulk": (C.c_int, [C.c_void_p, C.c_int, PD]),
     "sfx_free_energy": (C.c_int, [C.c_void_p, PD]),
+    "sfx_grand_potential": (C.c_int, [C.c_void_p, PD]),
+    "sfx_chemical_potential": (C.c_int, [C.c_void_p, C.c_int, PD]),
     "sfx_set_chi": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
     "sfx_solver_defaults": (C.c_int, [C.c_void_p]),
     "sfx_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
@@ -220,6 +222,16 @@ class System:
     def free_energy(self) -> float:
         v = C.c_double()
         check(lib().sfx_free_energy(self.h, C.byref(v)))
+        return v.value
+
+    def grand_potential(self) -> float:
+        v = C.c_double()
+        check(lib().sfx_grand_potential(self.h, C.byref(v)))
+        return v.value
+
+    def chemical_potential(self, mol: int) -> float:
+        v = C.c_double()
+        check(lib().sfx_chemical_potential(self.h, mol, C.byref(v)))
         return v.value
 
     def set_chi(self, a: int, b: int, value: float):

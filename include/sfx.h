@@ -171,6 +171,12 @@ int sfx_get_phi(sfx_system* s, int state, double* phi_out /* M */, int where);
 int sfx_get_lngn(sfx_system* s, int molecule, double* lngn);
 int sfx_get_phibulk(sfx_system* s, int molecule, double* phibulk);
 int sfx_free_energy(sfx_system* s, double* F);
+/* Grand potential (SF_MolList1stO.cpp:637-647 over the profile of :1001-1100)
+ * and chemical potential of a molecule (SF_MolList1stO.cpp:389-452), both of
+ * the last sfx_compute_phi / sfx_residual, for linear chains without charges,
+ * grafting or internal states; the melt form returns SFX_E_UNSUPPORTED. */
+int sfx_grand_potential(sfx_system* s, double* omega);
+int sfx_chemical_potential(sfx_system* s, int molecule, double* mu);
 /* Sets chi(a,b) = chi(b,a) = value for the following calls, keeping the
  * current iteration vector: one step of a `var` sweep over a chi parameter
  * (Vary.cpp:3-80; SF_Box.cpp:145-176 re-solves each step from the previous

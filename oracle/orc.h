@@ -61,6 +61,8 @@ void orc_sys_get_swf(const orc_sys* s, int state, double* out);
 double orc_sys_lngn(const orc_sys* s, int mol);
 double orc_sys_phibulk(const orc_sys* s, int mol);
 double orc_sys_free_energy(orc_sys* s);
+double orc_sys_chemical_potential(const orc_sys* s, int mol);
+double orc_sys_grand_potential(orc_sys* s);
 /* Homogeneous initial guess (x = 0 everywhere). */
 int orc_sys_solve_diis(orc_sys* s, double* x, double tol, int max_iter, int m, double eta,
                        int* iterations, double* final_norm);

@@ -48,6 +48,10 @@ def lib():
         L.orc_sys_phibulk.argtypes = [C.c_void_p, C.c_int]
         L.orc_sys_free_energy.restype = D
         L.orc_sys_free_energy.argtypes = [C.c_void_p]
+        L.orc_sys_chemical_potential.restype = D
+        L.orc_sys_chemical_potential.argtypes = [C.c_void_p, C.c_int]
+        L.orc_sys_grand_potential.restype = D
+        L.orc_sys_grand_potential.argtypes = [C.c_void_p]
         L.orc_sys_solve_diis.argtypes = [C.c_void_p, PD, D, C.c_int, C.c_int, D, PI, PD]
         L.orc_mps_new.restype = C.c_void_p
         L.orc_mps_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, D, D, D, D]
@@ -177,6 +181,12 @@ class System:
 
     def free_energy(self):
         return lib().orc_sys_free_energy(self.h)
+
+    def chemical_potential(self, mol):
+        return lib().orc_sys_chemical_potential(self.h, mol)
+
+    def grand_potential(self):
+        return lib().orc_sys_grand_potential(self.h)
 
     def solve_diis(self, x, tol=1e-7, max_iter=1000, m=10, eta=0.1):
         x = _f64(x).copy()

@@ -15,6 +15,11 @@
  *   GetResiduals                SF_Solve.cpp:527-591
  *   GetFreeEnergy               SF_MolList1stO.cpp:606-635, :815-940
  *   chemIntRef                  SF_MolStructure.cpp:412-424
+ *   chemical potential          SF_MolList1stO.cpp:389-452 (first-generation
+ *                               freedoms, one state per segment, no reactions)
+ *   grand potential             SF_MolList1stO.cpp:637-647 over :1001-1100
+ *                               (no charges, no grafted chains, no external
+ *                               potential)
  * The melt form (SFX_FORM_MELT) replaces the solvent closure by explicit bulk
  * fractions and the free energy by MPS.cpp:253-261.
  */
@@ -548,6 +553,71 @@ double orc_sys_free_energy(orc_sys* s) {
   free(con);
   free(fe);
   return F;
+}
+
+/* Chemical potential of molecule m at the state of the last evaluation
+ * (SF_MolList1stO.cpp:389-452): ln phi_b + 1 - N sum_m' phi_b,m'/N_m'
+ * + N (sum_t phi_ref,t ChemIntBulk_t - chemIntRef) - N/2 sum_t phi_b,t ChemIntBulk_t,
+ * with phi_ref,t the fraction of the chain's segments in state t. */
+double orc_sys_chemical_potential(const orc_sys* s, int m) {
+  const omol* o = &s->mols[m];
+  double N = o->N;
+  double v = log(o->phib) + 1.0;
+  for (int k = 0; k < s->nm; k++) v -= N * (s->mols[k].phib / s->mols[k].N);
+  double d = 0;
+  for (int t = 0; t < s->ns; t++) {
+    int cnt = 0;
+    for (int k = 0; k < o->N; k++) cnt += o->seq[k] == t;
+    if (cnt) d += ((double)cnt / o->N) * chem_int_bulk(s, t);
+  }
+  v += N * (d - o->chemIntRef);
+  d = 0;
+  for (int t = 0; t < s->ns; t++) d += s->phib_state[t] * chem_int_bulk(s, t);
+  v -= 0.5 * N * d;
+  return v;
+}
+
+/* Grand potential of the last evaluation (SF_MolList1stO.cpp:637-647): the
+ * profile of :1001-1100 after SubtractBoundaries, weighted by lattice sites. */
+double orc_sys_grand_potential(orc_sys* s) {
+  long M = s->M;
+  int ns = s->ns;
+  double* w = calloc(M, sizeof(double));
+  for (int m = 0; m < s->nm; m++) {
+    const omol* o = &s->mols[m];
+    for (long z = 0; z < M; z++) {
+      double pm = 0;
+      for (int t = 0; t < ns; t++) pm += o->phi[t * M + z];
+      if (pm > 0) w[z] -= (pm - o->phib) / o->N;
+    }
+  }
+  for (int t = 0; t < ns; t++) {
+    if (s->frozen[t]) continue;
+    double cb = chem_int_bulk(s, t);
+    for (long z = 0; z < M; z++) {
+      double G = s->swf[t * M + z], up = 0;
+      if (G > 0) up = -log(G) - chem_int(s, t, z) + cb;
+      w[z] -= up * s->phi[t * M + z];
+    }
+  }
+  for (int i = 0; i < ns; i++) {
+    if (s->frozen[i]) continue;
+    for (int j = 0; j < ns; j++) {
+      if (s->frozen[j]) continue;
+      double c = s->chi[i * ns + j];
+      if (c == 0) continue;
+      for (long z = 0; z < M; z++) {
+        w[z] -= 0.5 * c * s->phi[i * M + z] * s->rho[j * M + z];
+        if (s->phi[i * M + z] > 0) w[z] += 0.5 * c * s->phib_state[i] * s->phib_state[j];
+      }
+    }
+  }
+  orc_subtract_boundaries(s->L, w);
+  orc_multiply_sites(s->L, w);
+  double G = 0;
+  for (long z = 0; z < M; z++) G += w[z];
+  free(w);
+  return G;
 }
 
 /* DIIS of MPS.cpp:228-284 applied to the general residual. */

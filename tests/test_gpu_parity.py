@@ -349,3 +349,25 @@ def test_fourier_preconditioner_matches_numpy(sfx, n, n_a, n_b, chi):
     print(f"{n}^3 preconditioner vs numpy: {err:.2e}")
     assert err < 1e-4
     assert np.all(w[:, ~mask] == 0)
+
+
+@pytest.mark.parametrize("name", cases.NAMES)
+def test_grand_and_chemical_potentials(built, name, sfx):
+    """sfx_grand_potential and sfx_chemical_potential against the oracle at a
+    seeded field (FP64, 1e-10 relative); the melt form refuses both."""
+    lat, sysd, G, O = built[name]
+    x = cases.start(lat, sysd, seed=41, sigma=0.1)
+    G.residual(x, precision="fp64")
+    if sysd.form == "melt":
+        with pytest.raises(sfx.SFError):
+            G.grand_potential()
+        with pytest.raises(sfx.SFError):
+            G.chemical_potential(0)
+        return
+    O.residual(x)
+    og, oo = G.grand_potential(), O.grand_potential()
+    print(f"{name}: grand potential GPU {og:.12e} oracle {oo:.12e}")
+    assert abs(og - oo) <= 1e-10 * max(1.0, abs(oo))
+    for m in range(len(sysd.molecules)):
+        mg, mo = G.chemical_potential(m), O.chemical_potential(m)
+        assert abs(mg - mo) <= 1e-10 * max(1.0, abs(mo)), (m, mg, mo)
