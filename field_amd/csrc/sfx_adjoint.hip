@@ -467,10 +467,17 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
   auto dslot = [&](int i) { return dstore + (size_t)i * Mn; };
   const int N = mo.N;
   const ChainPlan& plan = mo.plan;
-  for (int t = 0; t < S.ns; t++)
-    if (mo.uses[t]) zero<float>(daccp(t), Mn, st);
-  zero<unsigned>(maxf, (S.maxN + 2) * kStepWords, st);
-  zero<unsigned>(maxb, (S.maxN + 2) * kStepWords, st);
+  {
+    ZeroList zl;
+    for (int t = 0; t < S.ns; t++)
+      if (mo.uses[t]) {
+        zero_list_add(zl, daccp(t), Mn, st);
+        if (S.middle_out(mo)) zero_list_add(zl, (float*)W.dacc2.p + ((size_t)m * S.ns + t) * Mn, Mn, st);
+      }
+    zero_list_add(zl, maxf, (long long)(S.maxN + 2) * kStepWords, st);
+    zero_list_add(zl, maxb, (long long)(S.maxN + 2) * kStepWords, st);
+    launch_zero_list(zl, st);
+  }
   auto fsc = [&](int step, int pair) { return RankScale{maxf, lnf, lnf, scal + MS_REF, step, pair}; };
   auto bsc = [&](int step, int pair) { return RankScale{maxb, lnb, lnf, scal + MS_REF, step, pair}; };
   auto state_of_rank = [&](int s) { return mo.seq[s - 1]; };
@@ -527,8 +534,7 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
     auto dacc2p = [&](int t) { return dacc2 + ((size_t)m * S.ns + t) * Mn; };
     float* qq[2] = {(float*)W.dpp2.p, (float*)W.dpp2.p + Mn};  // backward running rank
     float* dqq[2] = {(float*)W.dpp2.p + 2 * Mn, (float*)W.dpp2.p + 3 * Mn};
-    for (int t = 0; t < S.ns; t++)
-      if (mo.uses[t]) zero<float>(dacc2p(t), Mn, st);
+    // (the second accumulators were cleared with the first above)
     auto rank_of = [&](int k) { return N - k + 1; };
     auto bstate = [&](int k) { return mo.seq[rank_of(k) - 1]; };
     k_tinit<float><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0));
@@ -911,7 +917,10 @@ void jvp_at_base(System& S, const double* v, double* jv, int precision) {
   bool fp32 = precision == SFX_FP32;
   float* v32 = (float*)W.v32.p;
   if (fp32) k_state_dir<float><<<grid1(Mn, kTB), kTB, 0, st>>>(Mn, S.ns, S.d_st_itvar, v, v32);
-  if (fp32 && S.use_block3) launch_gmax(L, S.d_swf32, S.ns, S.d_gmax, st);
+  if (fp32 && S.use_block3 && !S.gmax_valid) {  // the base's FP32 SWF maxima, unless its compute_phi made them
+    launch_gmax(L, S.d_swf32, S.ns, S.d_gmax, st);
+    S.gmax_valid = true;
+  }
   for (int pass = 0; pass < 2; pass++)
     for (int m = 0; m < S.nm; m++) {
       bool solv = S.mols[m].freedom == SFX_SOLVENT;

@@ -88,6 +88,16 @@ struct SysParams {
   double mol_ref[SFX_MAXM];
 };
 
+// up to eight device arrays of 4-byte words cleared by one launch
+struct ZeroList {
+  void* p[8];
+  long long words[8];
+  int n = 0;
+};
+void launch_zero_list(const ZeroList& z, hipStream_t st);
+// adds an array, launching the list first when it is full
+void zero_list_add(ZeroList& z, void* p, long long words, hipStream_t st);
+
 template <typename T, typename A>
 void launch_prop(const DevLattice& L, const T* Gin, const T* g, T* Gout, const T* Gpair, A* acc,
                  const RankScale& sc, hipStream_t st);

@@ -4,6 +4,8 @@
 // reference layout but only interior sites are ever written or read (neighbour
 // coordinates are remapped, see sfx_device.h), so halo slots cost memory only.
 #include "sfx_device.h"
+#include <algorithm>
+
 #include "sfx_kernels.h"
 
 namespace sfx {
@@ -603,6 +605,38 @@ int launch_free_energy(const DevLattice& L, const SysParams& sp, const double* p
   unsigned nb = grid1(L.M, kBlock);
   k_free_energy<<<nb, kBlock, 0, st>>>(L, sp, phi, phi_m, rho, swf, u, molscal, partial);
   return (int)nb;
+}
+
+__global__ void k_zero_list(ZeroList z, long long total) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long off = i;
+    int k = 0;
+    while (k < z.n - 1 && off >= z.words[k]) {
+      off -= z.words[k];
+      k++;
+    }
+    ((unsigned*)z.p[k])[off] = 0u;
+  }
+}
+
+void launch_zero_list(const ZeroList& z, hipStream_t st) {
+  long long total = 0;
+  for (int k = 0; k < z.n; k++) total += z.words[k];
+  if (total <= 0) return;
+  const unsigned nb = (unsigned)std::min<long long>(grid1(total, kBlock), 4096);
+  k_zero_list<<<nb, kBlock, 0, st>>>(z, total);
+}
+
+void zero_list_add(ZeroList& z, void* p, long long words, hipStream_t st) {
+  if (words <= 0) return;
+  if (z.n == 8) {
+    launch_zero_list(z, st);
+    z.n = 0;
+  }
+  z.p[z.n] = p;
+  z.words[z.n] = words;
+  z.n++;
 }
 
 int launch_grand_potential(const DevLattice& L, const SysParams& sp, const double* phi, const double* phi_m,

@@ -204,6 +204,7 @@ struct System {
     return use_block3 && schedule == SFX_SCHEDULE_AUTO && mo.kind == 1 && !mo.plan.c && mo.N >= 2;
   }
   int schedule = SFX_SCHEDULE_AUTO;  // sfx_options.chain_schedule
+  bool gmax_valid = false;           // d_gmax holds the maxima of the current FP32 SWF
 };
 
 // Fourier-space Newton-Krylov preconditioner (sfx_precond.hip)

@@ -405,11 +405,22 @@ void System::sweep(int m, bool scaled) {
   };
   const int N = mo.N;
   const int P = sizeof(T) == 4 ? SFX_FP32 : SFX_FP64;
-  for (int t = 0; t < ns; t++)
-    if (mo.uses[t]) zero_bytes(accp(t), (size_t)Mn * sizeof(A), st);
-  if (scaled) {
-    k_fill_u32<<<grid1((N + 2) * kStepWords, 256), 256, 0, st>>>(maxf, 0u, (N + 2) * kStepWords);
-    k_fill_u32<<<grid1((N + 2) * kStepWords, 256), 256, 0, st>>>(maxb, 0u, (N + 2) * kStepWords);
+  {
+    // accumulators (also the backward chain's of a middle-out sweep) and the
+    // step maxima, cleared by one launch
+    ZeroList zl;
+    const long long aw = (long long)Mn * (long long)(sizeof(A) / 4);
+    for (int t = 0; t < ns; t++)
+      if (mo.uses[t]) {
+        zero_list_add(zl, accp(t), aw, st);
+        if constexpr (std::is_same<T, A>::value)
+          if (middle_out(mo)) zero_list_add(zl, (A*)d_acc2 + ((size_t)m * ns + t) * Mn, aw, st);
+      }
+    if (scaled) {
+      zero_list_add(zl, maxf, (long long)(N + 2) * kStepWords, st);
+      zero_list_add(zl, maxb, (long long)(N + 2) * kStepWords, st);
+    }
+    launch_zero_list(zl, st);
   }
   RankScale none{nullptr, nullptr, nullptr, nullptr, 0, 0};
   auto fsc = [&](int step, int pair) {
@@ -610,8 +621,7 @@ void System::sweep_blocked(int m, bool scaled) {
     T* acc2 = (T*)d_acc2;
     auto acc2p = [&](int t) { return acc2 + ((size_t)m * ns + t) * Mn; };
     T* qq[2] = {(T*)d_pp2, (T*)d_pp2 + Mn};
-    for (int t = 0; t < ns; t++)
-      if (mo.uses[t]) zero_bytes(acc2p(t), (size_t)Mn * sizeof(T), st);
+    // (the second accumulators were cleared with the first by sweep())
     auto rank_of = [&](int k) { return N - k + 1; };  // backward step k holds rank N-k+1
     auto bstate = [&](int k) { return mo.seq[rank_of(k) - 1]; };
     launch_init_rank<T>(L, gp(mo.seq[0]), slot(0), fsc(1, 0), st);
@@ -789,7 +799,11 @@ void System::sweep_blocked(int m, bool scaled) {
 void System::compute_phi(int prec) {
   const DevLattice& L = lat.d;
   launch_swf(L, d_u, d_itmask, ns, d_st_itvar, d_swf, d_swf32, stream);
-  if (prec == SFX_FP32 && use_block3) launch_gmax(L, d_swf32, ns, d_gmax, stream);
+  gmax_valid = false;
+  if (prec == SFX_FP32 && use_block3) {
+    launch_gmax(L, d_swf32, ns, d_gmax, stream);
+    gmax_valid = true;
+  }
   // non-solvent molecules first, the solvent closes phi_b (SF_MolList1stO.cpp:280-292)
   for (int pass = 0; pass < 2; pass++)
     for (int m = 0; m < nm; m++) {
