@@ -394,17 +394,30 @@ def calculations(inp: Input):
             yield (c, x) + build(sub, c)
 
 
-def run(path_or_text: str, **solver_overrides):
+def output_names(inp: Input, calc: int = 0) -> Dict[str, object]:
+    """Names the output lines of a calculation carry: its sys and newton
+    names ('noname' when absent) and its molecules."""
+    sysn = inp.names(calc, "sys")
+    newn = inp.names(calc, "newton")
+    return {"sys": sysn[0] if sysn else "noname", "newton": newn[0] if newn else "noname",
+            "mol": inp.names(calc, "mol")}
+
+
+def run(path_or_text: str, kal: Optional[str] = None, kal_template: Optional[str] = None, **solver_overrides):
     """Solve every calculation of an SFbox input on the GPU. A calculation
     starts from the previous solution when the iteration vector keeps its
     size, as the reference's default initial guess does, and from u = 0
     otherwise. Returns one dict per calculation: var value, convergence,
     iterations, residual norm, free energy, grand potential, per molecule
     ln G_N, phi_b and the chemical potential, and the solution x with the
-    lattice and system it belongs to."""
+    lattice and system it belongs to. With `kal`, the scalar results also go
+    to that file as a kal table (field_amd.output.write_kal), one row per
+    calculation, filtered by `kal_template` (input syntax) when given."""
     from . import sfx
     text = open(path_or_text).read() if os.path.exists(path_or_text) else path_or_text
     inp = Input(text)
+    if kal is not None:
+        from .output import write_kal
     results, x = [], None
     for c, v, lat, sysd, solver in calculations(inp):
         opts = {**solver, **solver_overrides}
@@ -423,4 +436,6 @@ def run(path_or_text: str, **solver_overrides):
             "chemical_potential": [G.chemical_potential(m) for m in range(nm)],
         })
         G.close()
+        if kal is not None:  # one row per calculation, appended as it finishes
+            write_kal(kal, results[-1:], output_names(inp, c), kal_template, append=len(results) > 1)
     return results

@@ -37,12 +37,17 @@ start
 """
 
 
-def test_var_sweep_input(orc):
+def test_var_sweep_input(orc, tmp_path):
     """phi_b from 1e-3 to 1e-1 in decades, each step warm-started from the
     last: every solution passes the oracle's residual, and the free energy,
-    grand potential and chemical potentials match the oracle's there."""
-    res = run(SWEEP)
+    grand potential and chemical potentials match the oracle's there. The kal
+    table holds a header and one row per step."""
+    kal = tmp_path / "sweep.kal"
+    res = run(SWEEP, kal=str(kal))
     assert [r["var"] for r in res] == pytest.approx([1e-3, 1e-2, 1e-1])
+    rows = kal.read_text().splitlines()
+    assert len(rows) == 4 and rows[0].startswith("sys : noname : free energy")
+    assert [float(r.split("\t")[0]) for r in rows[1:]] == pytest.approx([r["free_energy"] for r in res], rel=1e-12)
     for r in res:
         assert r["converged"], r
         O = orc.System(r["lattice"], r["system"])

@@ -128,3 +128,28 @@ def test_var_linear_and_exponential():
     assert typ == "integer" and vals == [10, 15, 20, 22]
     sizes = [tuple(c[2].n_layers) for c in calculations(Input(it))]
     assert sizes == [(10,), (15,), (20,), (22,)]
+
+
+def test_kal_table_and_template():
+    """The kal table (Output.cpp:640-718): a header of `class : name :
+    property` keys and one tab-separated row per calculation; a template in
+    input syntax, '*' allowed, keeps the matching lines (the reference's
+    pro.tmp template style)."""
+    from field_amd.output import kal_text
+    from field_amd.sfbox_input import output_names
+    names = output_names(Input(WALL))
+    assert names == {"sys": "noname", "newton": "isaac", "mol": ["pol", "sol"]}
+    res = [{"free_energy": -1.5, "grand_potential": 0.25, "phibulk": [0.01, 0.99], "lngn": [3.0, 0.0],
+            "chemical_potential": [-2.0, -0.5], "iterations": 12, "residual_norm": 1e-9, "converged": True},
+           {"free_energy": -1.25, "grand_potential": 0.5, "phibulk": [0.02, 0.98], "lngn": [3.5, 0.0],
+            "chemical_potential": [-1.5, -0.25], "iterations": 7, "residual_norm": 2e-9, "converged": False}]
+    tmpl = "sys : * : free energy : 1\nmol : pol : FH-MU : 1\nnewton : * : error occurred : false\n"
+    text = kal_text(res, names, tmpl)
+    assert text.splitlines() == [
+        "sys : noname : free energy\tmol : pol : FH-MU\tnewton : isaac : error occurred",
+        "-1.5\t-2\tfalse",
+        "-1.25\t-1.5\ttrue"]
+    full = kal_text(res, names).splitlines()
+    assert full[0].split("\t")[:3] == ["sys : noname : free energy", "sys : noname : grand potential",
+                                       "mol : pol : phibulk"]
+    assert len(full) == 3 and len(full[1].split("\t")) == 2 + 3 * 2 + 3
