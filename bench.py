@@ -20,7 +20,6 @@ Prints one JSON line.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time

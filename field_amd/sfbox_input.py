@@ -36,7 +36,7 @@ from __future__ import annotations
 import math
 import os
 import re
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
@@ -113,7 +113,6 @@ class Input:
 
 
 _BOUNDS = ("surface", "mirror1", "mirror2", "periodic")
-_IGNORED_CLASSES = ("output", "alias", "reaction")  # not physics the solvers take
 
 
 def _real(v: str, what: str) -> float:
