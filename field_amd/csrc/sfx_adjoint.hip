@@ -77,8 +77,23 @@ __device__ __forceinline__ T vdir(const DevLattice& L, const T* v, int x, int y,
 
 // rank 1: G = g, dG = -v g
 template <typename T>
-__global__ void k_tinit(DevLattice L, const T* __restrict__ g, const T* __restrict__ v, T* __restrict__ G,
-                        T* __restrict__ dG, RankScale sc) {
+struct TInit {
+  const T* g;
+  const T* v;
+  T* G;
+  T* dG;
+  RankScale sc;
+};
+
+// rank 1 of one chain, or of two (blockIdx.y) when a middle-out sweep seeds both
+template <typename T>
+__global__ void k_tinit2(DevLattice L, TInit<T> c0, TInit<T> c1) {
+  const TInit<T>& c = blockIdx.y ? c1 : c0;
+  const T* __restrict__ g = c.g;
+  const T* __restrict__ v = c.v;
+  T* __restrict__ G = c.G;
+  T* __restrict__ dG = c.dG;
+  const RankScale& sc = c.sc;
   long long q = site_ordinal();
   float m = 0.f;
   if (q < L.V) {
@@ -101,6 +116,16 @@ __global__ void k_tinit(DevLattice L, const T* __restrict__ g, const T* __restri
       if (blockIdx.x == 0) sc.lnv[sc.step] = 0.0;
     }
   }
+}
+
+template <typename T>
+static void tinit(const DevLattice& L, const T* g, const T* v, T* G, T* dG, const RankScale& sc, hipStream_t st) {
+  const TInit<T> c{g, v, G, dG, sc};
+  k_tinit2<T><<<dim3(grid1(L.V, kTB), 1), kTB, 0, st>>>(L, c, c);
+}
+template <typename T>
+static void tinit2(const DevLattice& L, const TInit<T>& c0, const TInit<T>& c1, hipStream_t st) {
+  k_tinit2<T><<<dim3(grid1(L.V, kTB), 2), kTB, 0, st>>>(L, c0, c1);
 }
 
 // G' = g A[G] s, dG' = g (A[dG] - v A[G]) s; optional composition of
@@ -305,24 +330,28 @@ __global__ void k_tmol_finalize(MolParams mp, const double* p1, const double* p2
 
 // dphi_{m,t} at interior sites
 template <typename T>
-__global__ void k_tmol_phi(DevLattice L, const double* __restrict__ phi_m, const T* __restrict__ dacc,
-                           const T* __restrict__ dacc2, const double* __restrict__ g, const double* __restrict__ v,
-                           const double* __restrict__ scal, const double* __restrict__ dm, int monomer,
+__global__ void k_tmol_phi(DevLattice L, unsigned uses, int ns, const double* __restrict__ phi_m,
+                           const T* __restrict__ dacc, const T* __restrict__ dacc2, const double* __restrict__ g,
+                           const double* __restrict__ v, const double* __restrict__ dm, int monomer,
                            double* __restrict__ dphi_m) {
   long long q = site_ordinal();
   if (q >= L.V) return;
   int x, y, z;
   interior_coords(L, q, x, y, z);
-  long long r = sidx(L, x, y, z);
-  double ph = phi_m[r];
-  double rel = dm[1];
-  double gv = g[r];
-  double out;
-  if (monomer)
-    out = ph * (rel - v[r]);
-  else
-    out = gv > 0 ? ph * (rel + v[r]) + dm[3] * ((double)dacc[r] + (dacc2 ? (double)dacc2[r] : 0.0)) / gv : 0.0;
-  dphi_m[r] = out;
+  const long long r0 = sidx(L, x, y, z);
+  const double rel = dm[1];
+  for (int t = 0; t < ns; t++) {  // every state of the molecule (bit t of uses), arrays [state][M]
+    if (!((uses >> t) & 1u)) continue;
+    const long long r = (long long)t * L.M + r0;
+    const double ph = phi_m[r];
+    const double gv = g[r];
+    double out;
+    if (monomer)
+      out = ph * (rel - v[r]);
+    else
+      out = gv > 0 ? ph * (rel + v[r]) + dm[3] * ((double)dacc[r] + (dacc2 ? (double)dacc2[r] : 0.0)) / gv : 0.0;
+    dphi_m[r] = out;
+  }
 }
 
 // tangent totals (frozen states have no tangent); halos from their sources
@@ -398,13 +427,16 @@ __global__ void k_jres(DevLattice L, SysParams sp, const unsigned char* __restri
 }
 
 // per-state direction v_t = v_{itvar(t)} (zero for frozen states), in T
-template <typename T>
-__global__ void k_state_dir(long long M, int ns, const int* st_itvar, const double* v, T* out) {
+// (out32 optional: the same directions rounded to FP32)
+__global__ void k_state_dir(long long M, int ns, const int* st_itvar, const double* v, double* out,
+                            float* out32) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M) return;
   for (int t = 0; t < ns; t++) {
     int iv = st_itvar[t];
-    out[(long long)t * M + i] = iv >= 0 ? (T)v[(long long)iv * M + i] : T(0);
+    const double d = iv >= 0 ? v[(long long)iv * M + i] : 0.0;
+    out[(long long)t * M + i] = d;
+    if (out32) out32[(long long)t * M + i] = (float)d;
   }
 }
 
@@ -537,9 +569,9 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
     // (the second accumulators were cleared with the first above)
     auto rank_of = [&](int k) { return N - k + 1; };
     auto bstate = [&](int k) { return mo.seq[rank_of(k) - 1]; };
-    k_tinit<float><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0));
     const int tN = mo.seq[N - 1];
-    k_tinit<float><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(tN), vp(tN), slot(N - 1), dslot(N - 1), bsc(1, N));
+    tinit2<float>(L, TInit<float>{gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0)},
+                  TInit<float>{gp(tN), vp(tN), slot(N - 1), dslot(N - 1), bsc(1, N)}, st);
     const auto F1 = chain_launches(2, h, 2, state_of_rank);
     const auto B1 = chain_launches(2, N - h, 2, bstate);
     for (size_t i = 0; i < std::max(F1.size(), B1.size()); i++) {
@@ -614,7 +646,7 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
     k_tmol_finalize<<<1, 256, 0, st>>>(mo.params(m), S.d_partial, W.partial2.p, (int)nb, S.d_molscal, W.dmol.p, S.nm);
     return;
   }
-  k_tinit<float><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0));
+  tinit<float>(L, gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0), st);
   {
     int prev = 0, parity = 0;
     chain_blocks(2, plan.n, 2, state_of_rank, nosplit, [&](int sa, int sb) {
@@ -666,7 +698,7 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
   if (mo.kind == 1) {
     k_set_ref2<<<1, 1, 0, st>>>(scal, lnf, N, 1.0);
     const int tN = mo.seq[N - 1];
-    k_tinit<float><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(tN), vp(tN), pp[0], dpp[0], bsc(1, N));
+    tinit<float>(L, gp(tN), vp(tN), pp[0], dpp[0], bsc(1, N), st);
     const int iN = get(N);
     k_tconnect<float, true><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[0], dpp[0], slot(iN), dslot(iN), daccp(tN),
                                                              bsc(1, N), 1.0);
@@ -816,7 +848,7 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
   } else {
     auto slot = [&](int i) { return store + (size_t)i * Mn; };
     auto dslot = [&](int i) { return dstore + (size_t)i * Mn; };
-    k_tinit<T><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0));
+    tinit<T>(L, gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0), st);
     auto step = [&](int s, int src, int dst) {
       S.prof.begin(2, P, st);
       tprop<T>(L, slot(src), dslot(src), gp(mo.seq[s - 1]), vp(mo.seq[s - 1]), slot(dst), dslot(dst), nullptr,
@@ -828,7 +860,7 @@ static void tsweep(System& S, Tangent& W, int m, bool scaled, const T* vt, const
     if (mo.kind == 1) {
       k_set_ref2<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
       int tN = mo.seq[N - 1];
-      k_tinit<T><<<grid1(L.V, kTB), kTB, 0, st>>>(L, gp(tN), vp(tN), pp[0], dpp[0], bsc(1, N));
+      tinit<T>(L, gp(tN), vp(tN), pp[0], dpp[0], bsc(1, N), st);
       const int iN = rd.get(N);
       if (scaled)
         k_tconnect<T, true><<<grid1(L.V, kTB), kTB, 0, st>>>(L, pp[0], dpp[0], slot(iN), dslot(iN), daccp(tN),
@@ -890,14 +922,10 @@ static void tmol_phi(System& S, Tangent& W, int m, const double* v64state) {
   // the FP32 blocked tangent sweep of a middle-out chain leaves half of the
   // composition in the second accumulator
   const bool two = sizeof(T) == 4 && S.middle_out(mo);
-  for (int t = 0; t < S.ns; t++) {
-    if (!mo.uses[t]) continue;
-    size_t off = ((size_t)m * S.ns + t) * Mn;
-    k_tmol_phi<T><<<grid1(L.V, kTB), kTB, 0, S.stream>>>(
-        L, S.d_phi_m + off, (const T*)W.dacc.p + off, two ? (const T*)W.dacc2.p + off : nullptr,
-        S.d_swf + (size_t)t * Mn, v64state + (size_t)t * Mn, S.d_molscal + (size_t)m * MS_COUNT, W.dmol.p + m * 4,
-        mo.kind == 0, W.dphi_m.p + off);
-  }
+  const size_t off = (size_t)m * S.ns * Mn;
+  k_tmol_phi<T><<<grid1(L.V, kTB), kTB, 0, S.stream>>>(
+      L, mo.uses_mask(), S.ns, S.d_phi_m + off, (const T*)W.dacc.p + off, two ? (const T*)W.dacc2.p + off : nullptr,
+      S.d_swf, v64state, W.dmol.p + m * 4, mo.kind == 0, W.dphi_m.p + off);
 }
 
 void tangent_free(Tangent* t) { delete t; }
@@ -913,10 +941,9 @@ void jvp_at_base(System& S, const double* v, double* jv, int precision) {
   const long long Mn = L.M;
   hipStream_t st = S.stream;
   double* v64 = W.v64.p;
-  k_state_dir<double><<<grid1(Mn, kTB), kTB, 0, st>>>(Mn, S.ns, S.d_st_itvar, v, v64);
   bool fp32 = precision == SFX_FP32;
   float* v32 = (float*)W.v32.p;
-  if (fp32) k_state_dir<float><<<grid1(Mn, kTB), kTB, 0, st>>>(Mn, S.ns, S.d_st_itvar, v, v32);
+  k_state_dir<<<grid1(Mn, kTB), kTB, 0, st>>>(Mn, S.ns, S.d_st_itvar, v, v64, fp32 ? v32 : nullptr);
   if (fp32 && S.use_block3 && !S.gmax_valid) {  // the base's FP32 SWF maxima, unless its compute_phi made them
     launch_gmax(L, S.d_swf32, S.ns, S.d_gmax, st);
     S.gmax_valid = true;

@@ -103,6 +103,15 @@ void launch_prop(const DevLattice& L, const T* Gin, const T* g, T* Gout, const T
                  const RankScale& sc, hipStream_t st);
 template <typename T>
 void launch_init_rank(const DevLattice& L, const T* g, T* out, const RankScale& sc, hipStream_t st);
+// rank 1 of two chains (the forward and backward ends of a middle-out sweep) in one launch
+template <typename T>
+struct InitRank {
+  const T* g;
+  T* out;
+  RankScale sc;
+};
+template <typename T>
+void launch_init_rank2(const DevLattice& L, const InitRank<T>& c0, const InitRank<T>& c1, hipStream_t st);
 template <typename T, typename A>
 void launch_connect(const DevLattice& L, const T* a, const T* b, A* acc, const RankScale& sc, double half,
                     hipStream_t st);
@@ -117,15 +126,22 @@ void launch_mol_finalize(const MolParams& mp, const double* partial, int nparts,
 template <typename A>
 void launch_mol_phi(const DevLattice& L, const A* acc, const double* g, const double* scal, int monomer,
                     double* phi_m, hipStream_t st, const A* acc2 = nullptr);
+// phi_m of every state set in `uses` in one launch (state-major arrays from the given bases)
+template <typename A>
+void launch_mol_phi_states(const DevLattice& L, unsigned uses, int ns, const A* acc, const A* acc2, const double* g,
+                           const double* scal, double* phi_m, hipStream_t st);
 // scal[MS_REF] = lna[ia] + lnb[ib] (0 when lna is null): the reference log of
 // a middle-out sweep's compositions
 void launch_set_ref_pair(double* scal, const double* lna, int ia, const double* lnb, int ib, hipStream_t st);
 void launch_swf(const DevLattice& L, const double* u, const unsigned char* itmask, int nstates,
                 const int* st_itvar, double* swf, float* swf32, hipStream_t st);
+// sp != null: the same launch also writes the bulk fractions and ChemIntBulk
 void launch_phi_total(const DevLattice& L, int nstates, int nmol, const double* phi_m, const int* mol_uses,
-                      const double* fprof, const int* frozen, double* phi, double* phitot, hipStream_t st);
+                      const double* fprof, const int* frozen, double* phi, double* phitot, hipStream_t st,
+                      const SysParams* sp = nullptr, const double* molscal = nullptr, double* phib_state = nullptr,
+                      double* cbulk = nullptr);
 void launch_sides(const DevLattice& L, int nstates, const double* phi, double* rho, hipStream_t st);
-void launch_bulk(const SysParams& sp, const double* molscal, double* phib_state, double* cbulk, hipStream_t st);
+
 void launch_residual(const DevLattice& L, const SysParams& sp, const unsigned char* itmask, const double* swf,
                      const double* rho, const double* phitot, const double* cbulk, double* f, hipStream_t st);
 int launch_free_energy(const DevLattice& L, const SysParams& sp, const double* phi, const double* phi_m,

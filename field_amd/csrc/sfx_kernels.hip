@@ -73,8 +73,13 @@ __global__ void k_swf(DevLattice L, const double* __restrict__ u, const unsigned
 
 // ---------------------------------------------------------------------------
 // rank 1: G(1) = g (interior), optional running max for the FP32 log-scale
+// (blockIdx.y picks the chain when one launch seeds two)
 template <typename T>
-__global__ void k_init_rank(DevLattice L, const T* __restrict__ g, T* __restrict__ out, RankScale sc) {
+__global__ void k_init_rank(DevLattice L, InitRank<T> c0, InitRank<T> c1) {
+  const InitRank<T>& c = blockIdx.y ? c1 : c0;
+  const T* __restrict__ g = c.g;
+  T* __restrict__ out = c.out;
+  const RankScale& sc = c.sc;
   long long q = site_ordinal();
   float m = 0.f;
   if (q < L.V) {
@@ -271,11 +276,18 @@ void launch_set_ref_pair(double* scal, const double* lna, int ia, const double* 
 
 // state totals over molecules + frozen profiles, halos from their sources;
 // phitot on interior sites.
+__device__ void bulk_fractions(const SysParams& sp, const double* __restrict__ molscal, double* __restrict__ phib_state,
+                               double* __restrict__ cbulk);
+
+// state totals; the first thread also derives the bulk fractions and ChemIntBulk
+// from the molecules' phi_b (no dependence on the sites)
 __global__ void k_phi_total(DevLattice L, int nstates, int nmol, const double* __restrict__ phi_m,
                             const int* __restrict__ mol_uses, const double* __restrict__ fprof,
                             const int* __restrict__ frozen, double* __restrict__ phi,
-                            double* __restrict__ phitot) {
+                            double* __restrict__ phitot, SysParams sp, const double* __restrict__ molscal,
+                            double* __restrict__ phib_state, double* __restrict__ cbulk) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && phib_state) bulk_fractions(sp, molscal, phib_state, cbulk);
   if (i >= L.M) return;
   int x, y, z;
   site_coords(L, i, x, y, z);
@@ -321,9 +333,8 @@ __global__ void k_sides(DevLattice L, int nstates, const double* __restrict__ ph
 // ---------------------------------------------------------------------------
 // segment bulk fractions and ChemIntBulk (SF_MolStructure.cpp:605-612,
 // SF_SegmentList.cpp:610-623); one thread
-__global__ void k_bulk(SysParams sp, const double* __restrict__ molscal, double* __restrict__ phib_state,
-                       double* __restrict__ cbulk) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void bulk_fractions(const SysParams& sp, const double* __restrict__ molscal, double* __restrict__ phib_state,
+                               double* __restrict__ cbulk) {
   for (int t = 0; t < sp.ns; t++) {
     double pb = 0;
     if (sp.melt)
@@ -540,7 +551,13 @@ void launch_prop(const DevLattice& L, const T* Gin, const T* g, T* Gout, const T
 
 template <typename T>
 void launch_init_rank(const DevLattice& L, const T* g, T* out, const RankScale& sc, hipStream_t st) {
-  k_init_rank<T><<<grid1(L.V, kBlock), kBlock, 0, st>>>(L, g, out, sc);
+  const InitRank<T> c{g, out, sc};
+  k_init_rank<T><<<dim3(grid1(L.V, kBlock), 1), kBlock, 0, st>>>(L, c, c);
+}
+
+template <typename T>
+void launch_init_rank2(const DevLattice& L, const InitRank<T>& c0, const InitRank<T>& c1, hipStream_t st) {
+  k_init_rank<T><<<dim3(grid1(L.V, kBlock), 2), kBlock, 0, st>>>(L, c0, c1);
 }
 
 template <typename T, typename A>
@@ -575,24 +592,50 @@ void launch_mol_phi(const DevLattice& L, const A* acc, const double* g, const do
   k_mol_phi<A><<<grid1(L.V, kBlock), kBlock, 0, st>>>(L, acc, acc2, g, scal, monomer, phi_m);
 }
 
+// every state t set in `uses` (one bit per state): phi_m[t] = pref (acc[t] + acc2[t]) / g[t],
+// arrays [state][M] from the given bases
+template <typename A>
+__global__ void k_mol_phi_states(DevLattice L, unsigned uses, int ns, const A* __restrict__ acc,
+                                 const A* __restrict__ acc2, const double* __restrict__ g,
+                                 const double* __restrict__ scal, double* __restrict__ phi_m) {
+  long long q = site_ordinal();
+  if (q >= L.V) return;
+  int x, y, z;
+  interior_coords(L, q, x, y, z);
+  const long long r = sidx(L, x, y, z);
+  const double pref = scal[MS_PREF];
+  for (int t = 0; t < ns; t++) {
+    if (!((uses >> t) & 1u)) continue;
+    const long long i = (long long)t * L.M + r;
+    const double gv = g[i];
+    phi_m[i] = gv > 0 ? (((double)acc[i] + (acc2 ? (double)acc2[i] : 0.0)) / gv) * pref : 0.0;
+  }
+}
+
+template <typename A>
+void launch_mol_phi_states(const DevLattice& L, unsigned uses, int ns, const A* acc, const A* acc2, const double* g,
+                           const double* scal, double* phi_m, hipStream_t st) {
+  k_mol_phi_states<A><<<grid1(L.V, kBlock), kBlock, 0, st>>>(L, uses, ns, acc, acc2, g, scal, phi_m);
+}
+
 void launch_swf(const DevLattice& L, const double* u, const unsigned char* itmask, int nstates,
                 const int* st_itvar, double* swf, float* swf32, hipStream_t st) {
   k_swf<<<grid1(L.M, kBlock), kBlock, 0, st>>>(L, u, itmask, nstates, st_itvar, swf, swf32);
 }
 
 void launch_phi_total(const DevLattice& L, int nstates, int nmol, const double* phi_m, const int* mol_uses,
-                      const double* fprof, const int* frozen, double* phi, double* phitot, hipStream_t st) {
+                      const double* fprof, const int* frozen, double* phi, double* phitot, hipStream_t st,
+                      const SysParams* sp, const double* molscal, double* phib_state, double* cbulk) {
+  SysParams none{};
   k_phi_total<<<grid1(L.M, kBlock), kBlock, 0, st>>>(L, nstates, nmol, phi_m, mol_uses, fprof, frozen, phi,
-                                                     phitot);
+                                                     phitot, sp ? *sp : none, molscal, sp ? phib_state : nullptr,
+                                                     cbulk);
 }
 
 void launch_sides(const DevLattice& L, int nstates, const double* phi, double* rho, hipStream_t st) {
   k_sides<<<grid1(L.M, kBlock), kBlock, 0, st>>>(L, nstates, phi, rho);
 }
 
-void launch_bulk(const SysParams& sp, const double* molscal, double* phib_state, double* cbulk, hipStream_t st) {
-  k_bulk<<<1, 64, 0, st>>>(sp, molscal, phib_state, cbulk);
-}
 
 void launch_residual(const DevLattice& L, const SysParams& sp, const unsigned char* itmask, const double* swf,
                      const double* rho, const double* phitot, const double* cbulk, double* f, hipStream_t st) {
@@ -658,6 +701,10 @@ template void launch_prop<float, float>(const DevLattice&, const float*, const f
                                         const RankScale&, hipStream_t);
 template void launch_init_rank<double>(const DevLattice&, const double*, double*, const RankScale&, hipStream_t);
 template void launch_init_rank<float>(const DevLattice&, const float*, float*, const RankScale&, hipStream_t);
+template void launch_init_rank2<double>(const DevLattice&, const InitRank<double>&, const InitRank<double>&,
+                                        hipStream_t);
+template void launch_init_rank2<float>(const DevLattice&, const InitRank<float>&, const InitRank<float>&,
+                                       hipStream_t);
 template void launch_connect<double, double>(const DevLattice&, const double*, const double*, double*,
                                              const RankScale&, double, hipStream_t);
 template void launch_connect<float, float>(const DevLattice&, const float*, const float*, float*, const RankScale&,
@@ -670,5 +717,9 @@ template void launch_mol_phi<double>(const DevLattice&, const double*, const dou
                                      hipStream_t, const double*);
 template void launch_mol_phi<float>(const DevLattice&, const float*, const double*, const double*, int, double*,
                                     hipStream_t, const float*);
+template void launch_mol_phi_states<double>(const DevLattice&, unsigned, int, const double*, const double*,
+                                            const double*, const double*, double*, hipStream_t);
+template void launch_mol_phi_states<float>(const DevLattice&, unsigned, int, const float*, const float*,
+                                           const double*, const double*, double*, hipStream_t);
 
 }  // namespace sfx

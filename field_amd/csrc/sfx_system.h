@@ -92,6 +92,12 @@ struct Molecule {
   MolParams params(int index) const {
     return MolParams{index, N, freedom, kind, phib, theta};
   }
+  unsigned uses_mask() const {  // one bit per state the chain contains
+    unsigned u = 0;
+    for (size_t t = 0; t < uses.size(); t++)
+      if (uses[t]) u |= 1u << t;
+    return u;
+  }
 };
 
 // HIP-event timing of propagator launches (include/sfx_profile.h)

@@ -465,9 +465,8 @@ void System::sweep(int m, bool scaled) {
     }
     int nb = launch_wsum6<T>(L, pp6[(N - 1) & 1], nullptr, d_partial, nullptr, st);
     launch_mol_finalize(mo.params(m), d_partial, nb, scal, scaled ? lnb + N : nullptr, d_molscal, nm, st);
-    for (int t = 0; t < ns; t++)
-      if (mo.uses[t])
-        launch_mol_phi<A>(L, accp(t), d_swf + (size_t)t * Mn, scal, 0, d_phi_m + ((size_t)m * ns + t) * Mn, st);
+    launch_mol_phi_states<A>(L, mo.uses_mask(), ns, accp(0), (const A*)nullptr, d_swf, scal,
+                             d_phi_m + (size_t)m * ns * Mn, st);
     return;
   }
   if (mo.kind == 0) {
@@ -534,8 +533,8 @@ void System::sweep(int m, bool scaled) {
   }
   int nb = launch_wsum<T>(L, final_G, d_partial, st);
   launch_mol_finalize(mo.params(m), d_partial, nb, scal, lnfinal, d_molscal, nm, st);
-  for (int t = 0; t < ns; t++)
-    if (mo.uses[t]) launch_mol_phi<A>(L, accp(t), d_swf + (size_t)t * Mn, scal, 0, d_phi_m + ((size_t)m * ns + t) * Mn, st);
+  launch_mol_phi_states<A>(L, mo.uses_mask(), ns, accp(0), (const A*)nullptr, d_swf, scal, d_phi_m + (size_t)m * ns * Mn,
+                           st);
 }
 
 // The sweep of sweep() for a first-order chain (full or half store) on the
@@ -624,8 +623,8 @@ void System::sweep_blocked(int m, bool scaled) {
     // (the second accumulators were cleared with the first by sweep())
     auto rank_of = [&](int k) { return N - k + 1; };  // backward step k holds rank N-k+1
     auto bstate = [&](int k) { return mo.seq[rank_of(k) - 1]; };
-    launch_init_rank<T>(L, gp(mo.seq[0]), slot(0), fsc(1, 0), st);
-    launch_init_rank<T>(L, gp(mo.seq[N - 1]), slot(N - 1), bsc(1, N), st);
+    launch_init_rank2<T>(L, InitRank<T>{gp(mo.seq[0]), slot(0), fsc(1, 0)},
+                         InitRank<T>{gp(mo.seq[N - 1]), slot(N - 1), bsc(1, N)}, st);
     const auto F1 = chain_launches(2, h, 2, state_of_rank);
     const auto B1 = chain_launches(2, N - h, 2, bstate);
     for (size_t i = 0; i < std::max(F1.size(), B1.size()); i++) {
@@ -684,10 +683,8 @@ void System::sweep_blocked(int m, bool scaled) {
     }
     int nb = launch_wsum<T>(L, pp[fc], d_partial, st);  // forward rank N
     launch_mol_finalize(mo.params(m), d_partial, nb, scal, scaled ? lnf + N : nullptr, d_molscal, nm, st);
-    for (int t = 0; t < ns; t++)
-      if (mo.uses[t])
-        launch_mol_phi<T>(L, accp(t), d_swf + (size_t)t * Mn, scal, 0, d_phi_m + ((size_t)m * ns + t) * Mn, st,
-                          acc2p(t));
+    launch_mol_phi_states<T>(L, mo.uses_mask(), ns, accp(0), acc2p(0), d_swf, scal, d_phi_m + (size_t)m * ns * Mn,
+                             st);
     return;
   }
   // forward chain, ranks 2..plan.n; every rank kept (full store) or the
@@ -792,8 +789,8 @@ void System::sweep_blocked(int m, bool scaled) {
   }
   int nb = launch_wsum<T>(L, final_G, d_partial, st);
   launch_mol_finalize(mo.params(m), d_partial, nb, scal, lnfinal, d_molscal, nm, st);
-  for (int t = 0; t < ns; t++)
-    if (mo.uses[t]) launch_mol_phi<T>(L, accp(t), d_swf + (size_t)t * Mn, scal, 0, d_phi_m + ((size_t)m * ns + t) * Mn, st);
+  launch_mol_phi_states<T>(L, mo.uses_mask(), ns, accp(0), (const T*)nullptr, d_swf, scal, d_phi_m + (size_t)m * ns * Mn,
+                           st);
 }
 
 void System::compute_phi(int prec) {
@@ -814,9 +811,9 @@ void System::compute_phi(int prec) {
       else
         sweep<double, double>(m, false);
     }
-  launch_phi_total(L, ns, nm, d_phi_m, d_mol_uses, d_fprof, d_frozen, d_phi, d_phitot, stream);
+  launch_phi_total(L, ns, nm, d_phi_m, d_mol_uses, d_fprof, d_frozen, d_phi, d_phitot, stream, &sp, d_molscal,
+                   d_phib_state, d_cbulk);
   launch_sides(L, ns, d_phi, d_rho, stream);
-  launch_bulk(sp, d_molscal, d_phib_state, d_cbulk, stream);
   SFX_HIP(hipGetLastError());
   phi_valid = true;
   last_precision = prec;
