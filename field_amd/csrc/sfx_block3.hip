@@ -14,8 +14,8 @@
 // dacc += (dG_k G_pair + G_k dG_pair) c_k) in a register with the partner
 // values prefetched at launch start, and adds it to HBM once. The widened
 // ranks are computed by (y, z) columns, several threads per column along x;
-// consecutive threads take consecutive z, so LDS accesses are conflict-free
-// and HBM accesses coalesce along z.
+// consecutive threads take consecutive LDS words (whole rows), so LDS reads are
+// conflict-free, and HBM accesses coalesce along z.
 //
 // FP32 scaling: the input is divided by max(rank s0) and every rank by the
 // lattice maximum of g, so no rank exceeds 1 and at most nr ranks decay
@@ -196,15 +196,17 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Launch<T> la, in
   const T sixth = T(1.0 / 6.0);
   for (int k = 1; k <= nr; k++) {
     // rank k on the tile widened by e = nr - k: threads take (y, z) columns,
-    // several threads per column split it along x
+    // several threads per column split it along x. Consecutive threads cover
+    // whole LDS rows of kRZ (threads on the ring outside the widened tile
+    // idle), so a wave reads consecutive LDS words, free of bank conflicts.
     const int e = nr - k;
     const int lo = kH - e;
     const int bxn = kTX + 2 * e, byn = kTY + 2 * e, bzn = kTZ + 2 * e;
-    const int ncol = byn * bzn;
+    const int ncol = byn * kRZ;
     const int per = kB3 / ncol > 0 ? kB3 / ncol : 1;
-    if (t < per * ncol) {
-      const int c = t % ncol, xs = t / ncol;
-      const int base = (lo + c / bzn) * kRZ + (lo + c % bzn);
+    const int c = t % ncol, xs = t / ncol, RZ = c % kRZ;
+    if (t < per * ncol && RZ >= lo && RZ < lo + bzn) {
+      const int base = (lo + c / kRZ) * kRZ + RZ;
       for (int X = lo + xs; X < lo + bxn; X += per) {
         const int idx = X * kRYZ + base;
         const T s = (src[idx - kRYZ] + src[idx + kRYZ] + src[idx - kRZ] + src[idx + kRZ] + src[idx - 1] +
