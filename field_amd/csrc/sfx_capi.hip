@@ -188,6 +188,13 @@ int sfx_set_chi(sfx_system* s, int a, int b, double value) {
   });
 }
 
+int sfx_stream_set(sfx_system* s, void* stream) {
+  return guarded([&] {
+    SFX_REQUIRE(s, SFX_E_INVALID, "NULL argument");
+    s->s.set_stream((hipStream_t)stream);
+  });
+}
+
 int sfx_solver_defaults(sfx_solver_opts* o) {
   if (!o) return SFX_E_INVALID;
   std::memset(o, 0, sizeof *o);

@@ -47,7 +47,7 @@ struct VecOps {
   double* partial = nullptr;
   double* dres = nullptr;
   double* hres = nullptr;  // pinned host words for results read back
-  void init(long long n, hipStream_t st);  // no-op when already set up for n on st
+  void init(long long n, hipStream_t st);  // allocates only when n changes
   ~VecOps();
   void dots(const DotList& dl, double* host_out);  // waits for the results
   void dots_dev(const DotList& dl, double* dout);  // results stay on the device

@@ -94,12 +94,12 @@ static __global__ void k_copy(const double* x, double* y, long long n) {
 }
 
 void VecOps::init(long long n_, hipStream_t st) {
-  if (partial && n == n_ && stream == st) return;
+  stream = st;
+  if (partial && n == n_) return;
   if (partial) (void)hipFree(partial);
   if (dres) (void)hipFree(dres);
   partial = dres = nullptr;
   n = n_;
-  stream = st;
   nblk = (int)std::min<long long>(grid1(n, kDotBlock), 1024);
   SFX_HIP(hipMalloc(&partial, sizeof(double) * kMaxDots * nblk));
   SFX_HIP(hipMalloc(&dres, sizeof(double) * kMaxDots));

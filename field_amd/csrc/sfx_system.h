@@ -140,7 +140,8 @@ struct System {
   bool checkpointed = false;
   bool use_block3 = false;  // blocked 3-D simple-cubic sweeps (sfx_options.sweep_kernel)
   SysParams sp{};
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // where every call of this system runs
+  hipStream_t own_stream = nullptr;  // created with the system, used unless a caller sets one
 
   // device buffers
   double* d_u = nullptr;      // n_itvar * M
@@ -197,6 +198,9 @@ struct System {
   double grand_potential();
   double chemical_potential(int m);
   void sync();
+  // later calls run on st (nullptr: the system's own stream); work already
+  // queued on the previous stream is finished first
+  void set_stream(hipStream_t st);
 
   template <typename T, typename A>
   void sweep(int m, bool scaled);

@@ -70,7 +70,7 @@ System::~System() {
   tangent_free(tan);
   precond_free(precond);
   solve_work_free(work);
-  if (stream) (void)hipStreamDestroy(stream);
+  if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 
 static int is_palindrome(const std::vector<int>& s) {
@@ -246,7 +246,8 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
           std::count(mols[m].seq.begin(), mols[m].seq.end(), t) / (double)mols[m].N;
   }
   // device buffers
-  SFX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  SFX_HIP(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
+  stream = own_stream;
   const size_t M = (size_t)Mn;
   d_u = dalloc<double>(n_itvar * M);
   d_f = dalloc<double>(n_itvar * M);
@@ -318,6 +319,11 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
 }
 
 void System::sync() { SFX_HIP(hipStreamSynchronize(stream)); }
+
+void System::set_stream(hipStream_t st) {
+  sync();
+  stream = st ? st : own_stream;
+}
 
 void Profiler::begin(int kind, int prec, hipStream_t st, int ranks) {
   if (!on) return;

@@ -39,6 +39,7 @@ _SIGS = {
     "sfx_grand_potential": (C.c_int, [C.c_void_p, PD]),
     "sfx_chemical_potential": (C.c_int, [C.c_void_p, C.c_int, PD]),
     "sfx_set_chi": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
+    "sfx_stream_set": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sfx_solver_defaults": (C.c_int, [C.c_void_p]),
     "sfx_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "sfx_precondition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
@@ -237,6 +238,20 @@ class System:
         """chi(a,b) = chi(b,a) = value from here on (one step of a Vary.cpp
         sweep over a chi parameter; the caller re-solves from its last x)."""
         check(lib().sfx_set_chi(self.h, int(a), int(b), float(value)))
+
+    def set_stream(self, stream=None):
+        """Run the following calls on `stream` (a torch.cuda.Stream or a raw
+        hipStream_t as int; None: the system's own stream), after what this
+        system queued before (sfx_stream_set). Keep the stream alive while
+        the system uses it."""
+        if stream is None:
+            ptr = None
+        elif isinstance(stream, int):
+            ptr = stream
+        else:
+            ptr = stream.cuda_stream
+        self._stream = stream
+        check(lib().sfx_stream_set(self.h, C.c_void_p(ptr)))
 
     def profile_start(self):
         check(lib().sfx_profile_start(self.h))

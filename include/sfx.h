@@ -184,6 +184,15 @@ int sfx_chemical_potential(sfx_system* s, int molecule, double* mu);
  * variables (states with equal chi rows share one, SF_Solve.cpp:53-86). */
 int sfx_set_chi(sfx_system* s, int a, int b, double value);
 
+/* Runs the following calls of s on `stream` (a hipStream_t of the current
+ * device; NULL: the stream the system created for itself), after finishing
+ * what s queued on its previous stream. The caller keeps the stream alive
+ * while s uses it. Calls still wait for their own results before returning,
+ * as on the system's stream; the stream orders them after work the caller
+ * queued there. Replaces the reference GPU API's use of the default stream
+ * (Cuda_tools.cu:180-223 and on: every launch and cudaMemcpy on stream 0). */
+int sfx_stream_set(sfx_system* s, void* stream);
+
 /* ---- solvers ----------------------------------------------------------- */
 enum { SFX_SOLVER_DIIS = 0, SFX_SOLVER_NK = 1 };
 

@@ -261,6 +261,34 @@ def test_set_chi_equals_fresh_system(sfx, orc):
     assert np.array_equal(Ga.residual(x), fb)  # a refused change leaves chi alone
 
 
+def test_stream_set_gives_the_same_bits_on_a_caller_stream(sfx):
+    """sfx_stream_set: on a torch stream the system reads inputs torch queued
+    there and returns the residual, JVP and solve it returns on its own
+    stream, bit for bit; None hands it back its own stream."""
+    import torch
+
+    lat, sysd, x0 = configs.melt_3d(n=16)
+    G = sfx.System(lat, sysd, precision="fp32")
+    mask = np.tile(lat.interior_mask().reshape(-1), 2)
+    v = np.random.default_rng(11).normal(0, 1, G.nvar) * mask
+    f_own, jv_own = G.residual(x0), G.jvp(x0, v)
+    kw = dict(method="nk", tolerance=1e-8, max_iter=40, ptc_dt0=1.0)
+    x_own, st_own = G.solve(x0, **kw)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        G.set_stream(s)
+        xd = torch.from_numpy(x0).cuda() * 1.0  # queued on s
+        vd = torch.from_numpy(v).cuda() * 1.0
+        fd, jd = G.residual(xd), G.jvp(xd, vd)
+        xs, st_s = G.solve(xd, **kw)
+        s.synchronize()
+    assert np.array_equal(fd.cpu().numpy(), f_own)
+    assert np.array_equal(jd.cpu().numpy(), jv_own)
+    assert np.array_equal(xs.cpu().numpy(), x_own) and st_s["iterations"] == st_own["iterations"]
+    G.set_stream(None)
+    assert np.array_equal(G.residual(x0), f_own)
+
+
 def test_chi_sweep_reaches_melt_solution(sfx, orc):
     """A two-step `var` sweep of chi_AB (0.38, then 0.4) on the melt from its
     seeded start ends at a solution of the chi = 0.4 system (checked by the
