@@ -3,7 +3,8 @@
 // Newton step J(x) dx = -f(x) solved by restarted GMRES whose products J v are
 // the exact tangent-propagator JVPs (sfx_adjoint.hip), run in FP32 with
 // per-rank log-scaling. The residual f is evaluated with FP32 sweeps while
-// ||f|| is above fp64_switch and with FP64 sweeps below it; convergence is
+// ||f|| (measured, or predicted by the step's linear model) is above
+// fp64_switch and with FP64 sweeps below it; convergence is
 // declared on the FP64 residual only, so the FP32 Jacobian only steers the
 // step (iterative refinement). Backtracking on ||f|| globalises the step.
 // Where it applies, GMRES is right-preconditioned with the Fourier-space
@@ -91,7 +92,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     double eta = o.gmres_rtol;
     V.axpby(0.0, r, 0.0, dx);  // dx = 0
     int total = 0;
-    double beta0 = rn;
+    double beta0 = rn, lin = rn;  // lin: ||linear residual|| GMRES leaves
     for (int cycle = 0; cycle < 1 + o.gmres_max_iter / m && total < o.gmres_max_iter; cycle++) {
       // residual of the linear system: r_l = -r - J dx
       double beta;
@@ -102,6 +103,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
         op(dx, wv);
         V.axpby(-1.0, r, -1.0, wv);  // w = -r - J P z
         beta = std::sqrt(V.dot(wv, wv));
+        lin = beta;
         if (beta <= eta * beta0) break;
         V.axpby(1.0 / beta, wv, 0.0, Vk(0));
       }
@@ -180,12 +182,23 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
       vl.v[k] = dx;
       vl.c[k] = 1.0;
       V.combine(vl, dx);
+      lin = res;
       if (o.verbose) std::fprintf(stderr, "  gmres cycle=%d its=%d est=%.3e\n", cycle, j, res / beta0);
       if (res <= eta * beta0) break;
     }
     if (pc) {  // dx = P z
       V.copy(dx, wv);
       precond_apply(pc, S, wv, dx, sigma);
+    }
+    // The linear model predicts ||f|| at the full step: f + J dx, whose norm
+    // is the linear residual GMRES left plus, under continuation, at most
+    // sigma ||dx||. When that is already below the switch the trial point is
+    // evaluated in FP64 at once, rather than in FP32 first and then again in
+    // FP64 once the FP32 norm trips the switch.
+    if (prec == SFX_FP32 && lin < o.fp64_switch) {
+      double pred = lin;
+      if (sigma != 0.0) pred += sigma * std::sqrt(V.dot(dx, dx));
+      if (pred < o.fp64_switch) prec = SFX_FP64;
     }
     if (ptc) {
       V.copy(x, x0);

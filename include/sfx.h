@@ -206,7 +206,9 @@ typedef struct sfx_solver_opts {
   int gmres_max_iter;     /* NK: Krylov iterations per Newton step */
   double gmres_rtol;      /* NK: relative Krylov tolerance (forcing term) */
   int jvp_precision;      /* NK: precision of the tangent sweeps */
-  double fp64_switch;     /* NK: ||f|| below which residual sweeps run in FP64 */
+  double fp64_switch;     /* NK: ||f|| below which residual sweeps run in FP64;
+                             a trial point whose predicted ||f|| (the Newton
+                             step's linear model) is below it runs in FP64 too */
   int precondition;       /* NK: 1 = right-precondition GMRES with the Fourier-space Jacobian of the
                              homogeneous melt where it applies (periodic simple-cubic melt form);
                              0 = none. No counterpart in the reference (SFNewton has none). */
