@@ -248,12 +248,16 @@ int sfx_solve(sfx_system* s, const sfx_solver_opts* o, double* x, int where, sfx
 }
 
 // ---- kernel timing (include/sfx_profile.h) ----------------------------------
-int sfx_profile_start(sfx_system* s) {
+int sfx_profile_start(sfx_system* s) { return sfx_profile_start_select(s, ~0u); }
+
+int sfx_profile_start_select(sfx_system* s, unsigned select) {
   return guarded([&] {
     SFX_REQUIRE(s, SFX_E_INVALID, "NULL system");
     Profiler& p = s->s.prof;
     p.marks.clear();
     p.used = 0;
+    p.select = select;
+    p.open = false;
     p.on = true;
   });
 }

@@ -103,6 +103,9 @@ struct Molecule {
 // HIP-event timing of propagator launches (include/sfx_profile.h)
 struct Profiler {
   bool on = false;
+  unsigned select = ~0u;  // bit 2*kind + precision: that pair is timed
+  bool open = false;      // begin() recorded a start event that end() closes
+  bool wants(int kind, int prec) const { return on && (select >> (2 * kind + prec) & 1u); }
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   struct Mark {

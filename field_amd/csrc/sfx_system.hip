@@ -326,7 +326,8 @@ void System::set_stream(hipStream_t st) {
 }
 
 void Profiler::begin(int kind, int prec, hipStream_t st, int ranks) {
-  if (!on) return;
+  open = wants(kind, prec);
+  if (!open) return;
   while (pool.size() < used + 2) {
     hipEvent_t e;
     SFX_HIP(hipEventCreate(&e));
@@ -337,7 +338,7 @@ void Profiler::begin(int kind, int prec, hipStream_t st, int ranks) {
   used += 2;
 }
 std::pair<hipEvent_t, hipEvent_t> Profiler::events(int kind, int prec, int ranks) {
-  if (!on) return {nullptr, nullptr};
+  if (!wants(kind, prec)) return {nullptr, nullptr};
   while (pool.size() < used + 2) {
     hipEvent_t e;
     SFX_HIP(hipEventCreate(&e));
@@ -348,7 +349,8 @@ std::pair<hipEvent_t, hipEvent_t> Profiler::events(int kind, int prec, int ranks
   return {pool[used - 2], pool[used - 1]};
 }
 void Profiler::end(hipStream_t st) {
-  if (!on) return;
+  if (!open) return;
+  open = false;
   SFX_HIP(hipEventRecord(pool[marks.back().ev + 1], st));
 }
 Profiler::~Profiler() {

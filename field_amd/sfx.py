@@ -44,6 +44,7 @@ _SIGS = {
     "sfx_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "sfx_precondition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "sfx_profile_start": (C.c_int, [C.c_void_p]),
+    "sfx_profile_start_select": (C.c_int, [C.c_void_p, C.c_uint]),
     "sfx_profile_stop": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sfx_batch_create": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "sfx_batch_destroy": (C.c_int, [C.c_void_p]),
@@ -253,8 +254,16 @@ class System:
         self._stream = stream
         check(lib().sfx_stream_set(self.h, C.c_void_p(ptr)))
 
-    def profile_start(self):
-        check(lib().sfx_profile_start(self.h))
+    def profile_start(self, only=None):
+        """Time propagator launches with HIP events until profile_stop; `only`:
+        a list of (kernel kind, "fp32"/"fp64") pairs to time, the rest untimed."""
+        if only is None:
+            check(lib().sfx_profile_start(self.h))
+            return
+        sel = 0
+        for kind, prec in only:
+            sel |= 1 << (2 * kind + (1 if prec == "fp32" else 0))
+        check(lib().sfx_profile_start_select(self.h, C.c_uint(sel)))
 
     def profile_stop(self):
         """Per kernel kind and precision: launches, summed HIP-event ms, bytes per launch."""

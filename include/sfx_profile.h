@@ -45,6 +45,10 @@ typedef struct sfx_profile_report {
 } sfx_profile_report;
 
 int sfx_profile_start(sfx_system* s);
+/* As sfx_profile_start, timing only the (kind, precision) pairs whose bit
+ * 2*kind + precision is set in `select`; the other launches run without
+ * events (every event pair adds a few microseconds of queue time). */
+int sfx_profile_start_select(sfx_system* s, unsigned select);
 int sfx_profile_stop(sfx_system* s, sfx_profile_report* report);
 
 #ifdef __cplusplus

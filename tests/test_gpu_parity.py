@@ -289,6 +289,28 @@ def test_stream_set_gives_the_same_bits_on_a_caller_stream(sfx):
     assert np.array_equal(G.residual(x0), f_own)
 
 
+def test_profile_select_times_only_the_chosen_kernel(sfx):
+    """sfx_profile_start_select: the chosen (kind, precision) pair gets the
+    launches and ranks the full profile counts for it, and nothing else is
+    timed; results do not depend on profiling."""
+    lat, sysd, x0 = configs.melt_3d(n=16)
+    G = sfx.System(lat, sysd, precision="fp32")
+    v = np.random.default_rng(12).normal(0, 1, G.nvar) * np.tile(lat.interior_mask().reshape(-1), 2)
+    f0, j0 = G.residual(x0), G.jvp(x0, v)
+    G.profile_start()
+    f1, j1 = G.residual(x0), G.jvp(x0, v)
+    full = {(e["kind"], e["precision"]): e for e in G.profile_stop()}
+    assert (3, "fp32") in full and len(full) >= 3
+    G.profile_start(only=[(3, "fp32")])
+    f2, j2 = G.residual(x0), G.jvp(x0, v)
+    sel = G.profile_stop()
+    assert [(e["kind"], e["precision"]) for e in sel] == [(3, "fp32")]
+    assert sel[0]["launches"] == full[(3, "fp32")]["launches"] and sel[0]["ranks"] == full[(3, "fp32")]["ranks"]
+    assert sel[0]["total_ms"] > 0
+    for a, b in ((f0, f1), (f0, f2), (j0, j1), (j0, j2)):
+        assert np.array_equal(a, b)
+
+
 def test_chi_sweep_reaches_melt_solution(sfx, orc):
     """A two-step `var` sweep of chi_AB (0.38, then 0.4) on the melt from its
     seeded start ends at a solution of the chi = 0.4 system (checked by the
