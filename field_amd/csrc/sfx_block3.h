@@ -83,6 +83,8 @@ void launch_block3(const DevLattice& L, const B3Launch<T>& la, bool tangent, boo
 
 // max over the interior of each state's FP32 SWF, into step-max slots
 // gmax + t * kStepWords (zeroed here; read with get_step_max(.., 0))
-void launch_gmax(const DevLattice& L, const float* swf32, int ns, unsigned* gmax, hipStream_t st);
+// zeroed: gmax[0, ns * kStepWords) is already 0 on st (launch_swf can clear it)
+void launch_gmax(const DevLattice& L, const float* swf32, int ns, unsigned* gmax, hipStream_t st,
+                 bool zeroed = false);
 
 }  // namespace sfx

@@ -345,9 +345,9 @@ __global__ void __launch_bounds__(256) k_gmax(DevLattice L, const float* __restr
   }
 }
 
-void launch_gmax(const DevLattice& L, const float* swf32, int ns, unsigned* gmax, hipStream_t st) {
+void launch_gmax(const DevLattice& L, const float* swf32, int ns, unsigned* gmax, hipStream_t st, bool zeroed) {
   const int n = ns * kStepWords;
-  k_gmax_zero<<<grid1(n, 256), 256, 0, st>>>(gmax, n);
+  if (!zeroed) k_gmax_zero<<<grid1(n, 256), 256, 0, st>>>(gmax, n);
   const unsigned nb = (unsigned)std::min<long long>(256, std::max<long long>(1, L.V / 1024));
   k_gmax<<<dim3(nb, ns), 256, 0, st>>>(L, swf32, gmax);
   SFX_HIP(hipGetLastError());

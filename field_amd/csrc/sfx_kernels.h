@@ -133,8 +133,10 @@ void launch_mol_phi_states(const DevLattice& L, unsigned uses, int ns, const A* 
 // scal[MS_REF] = lna[ia] + lnb[ib] (0 when lna is null): the reference log of
 // a middle-out sweep's compositions
 void launch_set_ref_pair(double* scal, const double* lna, int ia, const double* lnb, int ib, hipStream_t st);
+// zero != null: the same launch also clears zero[0..nzero)
 void launch_swf(const DevLattice& L, const double* u, const unsigned char* itmask, int nstates,
-                const int* st_itvar, double* swf, float* swf32, hipStream_t st);
+                const int* st_itvar, double* swf, float* swf32, hipStream_t st, unsigned* zero = nullptr,
+                int nzero = 0);
 // sp != null: the same launch also writes the bulk fractions and ChemIntBulk
 void launch_phi_total(const DevLattice& L, int nstates, int nmol, const double* phi_m, const int* mol_uses,
                       const double* fprof, const int* frozen, double* phi, double* phitot, hipStream_t st,

@@ -53,8 +53,9 @@ __device__ double block_sum(double v) {
 // halos from their source site (SF_Solve.cpp:794-852 + SetBoundaries).
 __global__ void k_swf(DevLattice L, const double* __restrict__ u, const unsigned char* __restrict__ itmask,
                       int nstates, const int* __restrict__ st_itvar, double* __restrict__ swf,
-                      float* __restrict__ swf32) {
+                      float* __restrict__ swf32, unsigned* __restrict__ zero, int nzero) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nzero) zero[i] = 0u;
   if (i >= L.M) return;
   int x, y, z;
   site_coords(L, i, x, y, z);
@@ -619,8 +620,10 @@ void launch_mol_phi_states(const DevLattice& L, unsigned uses, int ns, const A* 
 }
 
 void launch_swf(const DevLattice& L, const double* u, const unsigned char* itmask, int nstates,
-                const int* st_itvar, double* swf, float* swf32, hipStream_t st) {
-  k_swf<<<grid1(L.M, kBlock), kBlock, 0, st>>>(L, u, itmask, nstates, st_itvar, swf, swf32);
+                const int* st_itvar, double* swf, float* swf32, hipStream_t st, unsigned* zero, int nzero) {
+  if (!zero) nzero = 0;
+  k_swf<<<grid1(std::max<long long>(L.M, nzero), kBlock), kBlock, 0, st>>>(L, u, itmask, nstates, st_itvar, swf,
+                                                                           swf32, zero, nzero);
 }
 
 void launch_phi_total(const DevLattice& L, int nstates, int nmol, const double* phi_m, const int* mol_uses,

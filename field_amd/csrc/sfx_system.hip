@@ -803,10 +803,11 @@ void System::sweep_blocked(int m, bool scaled) {
 
 void System::compute_phi(int prec) {
   const DevLattice& L = lat.d;
-  launch_swf(L, d_u, d_itmask, ns, d_st_itvar, d_swf, d_swf32, stream);
+  const bool gmax = prec == SFX_FP32 && use_block3;
+  launch_swf(L, d_u, d_itmask, ns, d_st_itvar, d_swf, d_swf32, stream, gmax ? d_gmax : nullptr, ns * kStepWords);
   gmax_valid = false;
-  if (prec == SFX_FP32 && use_block3) {
-    launch_gmax(L, d_swf32, ns, d_gmax, stream);
+  if (gmax) {
+    launch_gmax(L, d_swf32, ns, d_gmax, stream, true);
     gmax_valid = true;
   }
   // non-solvent molecules first, the solvent closes phi_b (SF_MolList1stO.cpp:280-292)
