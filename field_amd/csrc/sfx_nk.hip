@@ -27,30 +27,35 @@ void jvp_at_base(System& S, const double* v, double* jv, int precision);
 void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
   const long long n = (long long)S.n_itvar * S.lat.d.M;
   const int m = std::max(1, std::min(o.gmres_restart, 30));
-  SolveWork& W = solve_work(S, n, (size_t)m + 6);
+  Precond* pc = nullptr;
+  if (o.precondition && precond_supported(S)) {
+    if (!S.precond) S.precond = precond_create(S);
+    pc = S.precond;
+  }
+  // with a preconditioner, m more vectors keep P v_k of every Krylov vector
+  // and one the step P z built from them
+  SolveWork& W = solve_work(S, n, (size_t)m + 6 + (pc ? m + 1 : 0));
   VecOps& V = W.V;
   double* const wv = W.vec(m + 1);  // Arnoldi work vector
   double* const dx = W.vec(m + 2);
   double* const x0 = W.vec(m + 3);
   double* const r = W.vec(m + 4);
   double* const pz = W.vec(m + 5);
+  auto Zk = [&](int k) { return W.vec((size_t)m + 6 + k); };  // P v_k
+  double* const dxp = pc ? W.vec((size_t)2 * m + 6) : nullptr;
   double* const gs = W.gs.p;  // projections of both Gram-Schmidt passes, then ||w||^2
   constexpr int kGsPass = 32;
   double* x = S.d_u;
-  Precond* pc = nullptr;
-  if (o.precondition && precond_supported(S)) {
-    if (!S.precond) S.precond = precond_create(S);
-    pc = S.precond;
-  }
   const bool ptc = o.ptc_dt0 > 0;
   double dt = ptc ? o.ptc_dt0 : 0.0;
   double sigma = ptc ? 1.0 / dt : 0.0;
-  // the GMRES operator v -> (sigma I + J) P v (without a preconditioner P = I)
-  auto op = [&](const double* v, double* out) {
+  // the GMRES operator v -> (sigma I + J) P v (without a preconditioner P = I);
+  // P v goes to pv
+  auto op = [&](const double* v, double* out, double* pv) {
     const double* a = v;
     if (pc) {
-      precond_apply(pc, S, v, pz, sigma);
-      a = pz;
+      precond_apply(pc, S, v, pv, sigma);
+      a = pv;
     }
     jvp_at_base(S, a, out, o.jvp_precision);
     if (sigma != 0.0) V.axpby(sigma, a, 1.0, out);
@@ -91,6 +96,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     // ---- GMRES(m) for J dx = -r, dx0 = 0 ----
     double eta = o.gmres_rtol;
     V.axpby(0.0, r, 0.0, dx);  // dx = 0
+    if (pc) V.axpby(0.0, r, 0.0, dxp);
     int total = 0;
     double beta0 = rn, lin = rn;  // lin: ||linear residual|| GMRES leaves
     for (int cycle = 0; cycle < 1 + o.gmres_max_iter / m && total < o.gmres_max_iter; cycle++) {
@@ -100,7 +106,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
         V.axpby(-1.0 / rn, r, 0.0, Vk(0));
         beta = rn;
       } else {
-        op(dx, wv);
+        op(dx, wv, pz);
         V.axpby(-1.0, r, -1.0, wv);  // w = -r - J P z
         beta = std::sqrt(V.dot(wv, wv));
         lin = beta;
@@ -112,7 +118,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
       int j = 0;
       double res = beta;
       for (; j < m && total < o.gmres_max_iter; j++) {
-        op(Vk(j), wv);
+        op(Vk(j), wv, pc ? Zk(j) : pz);
         total++;
         // classical Gram-Schmidt, twice, with the projections kept on the
         // device; one read-back per Arnoldi step brings both passes and ||w||^2
@@ -182,14 +188,16 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
       vl.v[k] = dx;
       vl.c[k] = 1.0;
       V.combine(vl, dx);
+      if (pc) {  // the same combination of the P v_k: P dx without applying P again
+        for (int i = 0; i < k; i++) vl.v[i] = Zk(i);
+        vl.v[k] = dxp;
+        V.combine(vl, dxp);
+      }
       lin = res;
       if (o.verbose) std::fprintf(stderr, "  gmres cycle=%d its=%d est=%.3e\n", cycle, j, res / beta0);
       if (res <= eta * beta0) break;
     }
-    if (pc) {  // dx = P z
-      V.copy(dx, wv);
-      precond_apply(pc, S, wv, dx, sigma);
-    }
+    if (pc) V.copy(dxp, dx);  // dx = P z
     // The linear model predicts ||f|| at the full step: f + J dx, whose norm
     // is the linear residual GMRES left plus, under continuation, at most
     // sigma ||dx||. When that is already below the switch the trial point is
