@@ -39,7 +39,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
   double* const wv = W.vec(m + 1);  // Arnoldi work vector
   double* const dx = W.vec(m + 2);
   double* const x0 = W.vec(m + 3);
-  double* const r = W.vec(m + 4);
+  double* const r = S.d_f;  // the residual of the last evaluation (only eval() writes it)
   double* const pz = W.vec(m + 5);
   auto Zk = [&](int k) { return W.vec((size_t)m + 6 + k); };  // P v_k
   double* const dxp = pc ? W.vec((size_t)2 * m + 6) : nullptr;
@@ -74,7 +74,6 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
       st.residual_evals_fp32++;
     else
       st.residual_evals++;
-    V.copy(S.d_f, r);
     return std::sqrt(V.dot(r, r));
   };
   double rn = eval(prec);
