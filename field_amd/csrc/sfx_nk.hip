@@ -196,7 +196,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
       if (o.verbose) std::fprintf(stderr, "  gmres cycle=%d its=%d est=%.3e\n", cycle, j, res / beta0);
       if (res <= eta * beta0) break;
     }
-    if (pc) V.copy(dxp, dx);  // dx = P z
+    const double* const step = pc ? dxp : dx;  // P z
     // The linear model predicts ||f|| at the full step: f + J dx, whose norm
     // is the linear residual GMRES left plus, under continuation, at most
     // sigma ||dx||. When that is already below the switch the trial point is
@@ -204,12 +204,12 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     // FP64 once the FP32 norm trips the switch.
     if (prec == SFX_FP32 && lin < o.fp64_switch) {
       double pred = lin;
-      if (sigma != 0.0) pred += sigma * std::sqrt(V.dot(dx, dx));
+      if (sigma != 0.0) pred += sigma * std::sqrt(V.dot(step, step));
       if (pred < o.fp64_switch) prec = SFX_FP64;
     }
     if (ptc) {
       V.copy(x, x0);
-      V.axpby(1.0, dx, 1.0, x);
+      V.axpby(1.0, step, 1.0, x);
       const double rt = eval(prec);
       if (!std::isfinite(rt) || rt > 5.0 * rn) {
         V.copy(x0, x);
@@ -231,7 +231,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     bool accepted = false;
     for (int ls = 0; ls < 12; ls++) {
       V.copy(x0, x);
-      V.axpby(alpha, dx, 1.0, x);
+      V.axpby(alpha, step, 1.0, x);
       rt = eval(prec);
       if (std::isfinite(rt) && rt < best) {
         best = rt;
@@ -247,7 +247,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
       // take the best finite trial if it lowers ||f|| at all, else stop at x0
       V.copy(x0, x);
       if (best < rn) {
-        V.axpby(best_alpha, dx, 1.0, x);
+        V.axpby(best_alpha, step, 1.0, x);
         rt = eval(prec);
       } else {
         rt = eval(prec);
