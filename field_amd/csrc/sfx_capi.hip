@@ -112,10 +112,11 @@ int sfx_jvp(sfx_system* s, const double* x, const double* v, double* jv, int whe
     SFX_REQUIRE(s && x && v && jv, SFX_E_INVALID, "NULL argument");
     System& S = s->s;
     S.load_x(x, where);
-    DevBuf<double> vin((size_t)S.n_itvar * S.lat.d.M), out((size_t)S.n_itvar * S.lat.d.M);
-    load_vec(S, v, where, vin.p);
-    jvp_internal(S, vin.p, out.p, precision);
-    S.store_f(out.p, jv, where);
+    double *vin, *out;
+    S.abi_vectors(vin, out);
+    load_vec(S, v, where, vin);
+    jvp_internal(S, vin, out, precision);
+    S.store_f(out, jv, where);
   });
 }
 
@@ -124,10 +125,11 @@ int sfx_vjp(sfx_system* s, const double* x, const double* w, double* jtw, int wh
     SFX_REQUIRE(s && x && w && jtw, SFX_E_INVALID, "NULL argument");
     System& S = s->s;
     S.load_x(x, where);
-    DevBuf<double> win((size_t)S.n_itvar * S.lat.d.M), out((size_t)S.n_itvar * S.lat.d.M);
-    load_vec(S, w, where, win.p);
-    vjp_internal(S, win.p, out.p, precision);
-    S.store_f(out.p, jtw, where);
+    double *win, *out;
+    S.abi_vectors(win, out);
+    load_vec(S, w, where, win);
+    vjp_internal(S, win, out, precision);
+    S.store_f(out, jtw, where);
   });
 }
 
@@ -222,10 +224,11 @@ int sfx_precondition(sfx_system* s, const double* r, double* out, int where) {
     SFX_REQUIRE(precond_supported(S), SFX_E_UNSUPPORTED,
                 "the Fourier-space preconditioner needs a periodic simple-cubic melt");
     if (!S.precond) S.precond = precond_create(S);
-    DevBuf<double> a((size_t)S.nvar), b((size_t)S.nvar);
-    load_vec(S, r, where, a.p);
-    precond_apply(S.precond, S, a.p, b.p, 0.0);
-    S.store_f(b.p, out, where);
+    double *a, *b;
+    S.abi_vectors(a, b);
+    load_vec(S, r, where, a);
+    precond_apply(S.precond, S, a, b, 0.0);
+    S.store_f(b, out, where);
   });
 }
 

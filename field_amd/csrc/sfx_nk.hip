@@ -25,6 +25,8 @@ namespace sfx {
 void jvp_at_base(System& S, const double* v, double* jv, int precision);
 
 void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
+  SFX_REQUIRE(o.fp64_switch >= 0 && std::isfinite(o.fp64_switch), SFX_E_INVALID,
+              "fp64_switch must be finite and >= 0 (0: every residual in FP64)");
   const long long n = (long long)S.n_itvar * S.lat.d.M;
   const int m = std::max(1, std::min(o.gmres_restart, 30));
   Precond* pc = nullptr;
@@ -94,15 +96,17 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
     it++;
     // ---- GMRES(m) for J dx = -r, dx0 = 0 ----
     double eta = o.gmres_rtol;
-    V.axpby(0.0, r, 0.0, dx);  // dx = 0
-    if (pc) V.axpby(0.0, r, 0.0, dxp);
+    // cleared by stores, not by scaling: the work vectors are shared with
+    // DIIS and kept between solves, and 0 * NaN left there would stay NaN
+    V.zero(dx);
+    if (pc) V.zero(dxp);
     int total = 0;
     double beta0 = rn, lin = rn;  // lin: ||linear residual|| GMRES leaves
     for (int cycle = 0; cycle < 1 + o.gmres_max_iter / m && total < o.gmres_max_iter; cycle++) {
       // residual of the linear system: r_l = -r - J dx
       double beta;
       if (cycle == 0) {
-        V.axpby(-1.0 / rn, r, 0.0, Vk(0));
+        V.scale(-1.0 / rn, r, Vk(0));
         beta = rn;
       } else {
         op(dx, wv, pz);
@@ -110,7 +114,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
         beta = std::sqrt(V.dot(wv, wv));
         lin = beta;
         if (beta <= eta * beta0) break;
-        V.axpby(1.0 / beta, wv, 0.0, Vk(0));
+        V.scale(1.0 / beta, wv, Vk(0));
       }
       std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1, 0.0);
       g[0] = beta;
@@ -164,7 +168,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
         g[j + 1] = -sn[j] * g[j];
         g[j] = cs[j] * g[j];
         res = std::fabs(g[j + 1]);
-        if (hn > 0) V.axpby(1.0 / hn, wv, 0.0, Vk(j + 1));
+        if (hn > 0) V.scale(1.0 / hn, wv, Vk(j + 1));
         if (res <= eta * beta0 || hn == 0) {
           j++;
           break;
@@ -251,8 +255,16 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
         rt = eval(prec);
       } else {
         rt = eval(prec);
-        if (o.verbose) std::fprintf(stderr, "  line search found no descent, |f|=%.3e\n", rt);
         rn = rt;
+        if (prec == SFX_FP32) {
+          // FP32 residuals stop improving near their rounding floor; below it
+          // only FP64 evaluations can still show descent
+          prec = SFX_FP64;
+          rn = eval(prec);
+          if (o.verbose) std::fprintf(stderr, "  FP32 line search found no descent, continuing in FP64, |f|=%.3e\n", rn);
+          continue;
+        }
+        if (o.verbose) std::fprintf(stderr, "  line search found no descent, |f|=%.3e\n", rt);
         break;
       }
       if (o.verbose) std::fprintf(stderr, "  line search fell back to alpha=%g, |f|=%.3e\n", best_alpha, rt);

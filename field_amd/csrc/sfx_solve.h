@@ -57,6 +57,8 @@ struct VecOps {
   // y = v_last - sum_{k < count-1} dc[k] v_k, dc on the device (Gram-Schmidt)
   void subtract_projections(const VecList& vl, const double* dc, double* y);
   void axpby(double a, const double* x, double b, double* y);
+  void scale(double a, const double* x, double* y);  // y = a x (y's old contents never read)
+  void zero(double* y);                             // y = 0 (stores zeros, clears NaN/Inf)
   void sub(const double* x, const double* y, double* z);
   void copy(const double* x, double* y);
 };

@@ -177,6 +177,8 @@ struct System {
   double* d_cbulk = nullptr;
   double* d_scalar = nullptr;   // misc device scalars
   double* d_xabi = nullptr;     // nvar: ABI-layout staging
+  double* d_abi_in = nullptr;   // n_itvar * M each: the direction and the product of
+  double* d_abi_out = nullptr;  // sfx_jvp / sfx_vjp / sfx_precondition, allocated on first use
   int npart = 0;
   Tangent* tan = nullptr;
   Precond* precond = nullptr;  // Newton-Krylov preconditioner, built on first use
@@ -195,6 +197,9 @@ struct System {
   void load_x(const double* x, int where);
   // d_f -> f (ABI layout, host or device)
   void store_f(const double* src_internal, double* f, int where);
+  // the two internal-layout vectors of the product entry points (kept, so a
+  // call allocates nothing after the first)
+  void abi_vectors(double*& in, double*& out);
   void compute_phi(int precision);  // from d_u
   void residual_internal(int precision);  // d_u -> d_f
   double free_energy();

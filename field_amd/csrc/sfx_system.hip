@@ -64,7 +64,8 @@ static T* dalloc(size_t n) {
 System::~System() {
   void* ptrs[] = {d_u, d_f, d_swf, d_swf32, d_phi_m, d_acc, d_acc2, d_phi, d_rho, d_phitot, d_fprof, d_frozen,
                   d_st_itvar, d_mol_uses, d_itmask, d_sites, d_store, d_pp, d_pp2, d_molscal, d_lnf, d_lnb,
-                  d_maxf, d_maxb, d_partial, d_phib_state, d_cbulk, d_scalar, d_xabi, d_gmax};
+                  d_maxf, d_maxb, d_partial, d_phib_state, d_cbulk, d_scalar, d_xabi, d_gmax,
+                  d_abi_in, d_abi_out};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   tangent_free(tan);
@@ -372,6 +373,15 @@ void System::load_x(const double* x, int where) {
                                                                             n_itvar, Mn, d_u);
   }
   SFX_HIP(hipGetLastError());
+}
+
+void System::abi_vectors(double*& in, double*& out) {
+  if (!d_abi_in) {
+    d_abi_in = dalloc<double>((size_t)n_itvar * lat.d.M);
+    d_abi_out = dalloc<double>((size_t)n_itvar * lat.d.M);
+  }
+  in = d_abi_in;
+  out = d_abi_out;
 }
 
 // internal [itvar][M] vector -> ABI layout (host or device)

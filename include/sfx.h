@@ -208,7 +208,12 @@ typedef struct sfx_solver_opts {
   int jvp_precision;      /* NK: precision of the tangent sweeps */
   double fp64_switch;     /* NK: ||f|| below which residual sweeps run in FP64;
                              a trial point whose predicted ||f|| (the Newton
-                             step's linear model) is below it runs in FP64 too */
+                             step's linear model) is below it runs in FP64 too.
+                             Finite and >= 0 (0: FP64 throughout). FP32 residuals
+                             stop improving near their rounding floor (about
+                             1e-4 at 64^3); with a switch below it, an FP32 line
+                             search that finds no descent moves the solve to
+                             FP64 instead of ending it. */
   int precondition;       /* NK: 1 = right-precondition GMRES with the Fourier-space Jacobian of the
                              homogeneous melt where it applies (periodic simple-cubic melt form);
                              0 = none. No counterpart in the reference (SFNewton has none). */
