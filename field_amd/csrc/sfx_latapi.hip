@@ -2,11 +2,52 @@
 // counterparts of Cuda_tools.h:3-26 and of the Lat1stO virtual interface
 // (Lat1stO.h:19-50). One call = one kernel on the caller's stream.
 #include <cmath>
+#include <mutex>
 
 #include "sfx_device.h"
 #include "sfx_kernels.h"
 
 namespace sfx {
+
+// Scratch of sfx_lattice_lngn_noalloc: the per-workgroup partial sums of
+// launch_wsum (one per 256 interior sites) and, behind them, the total. One
+// copy per GPU, held until the process ends (512 KiB). Every call on a GPU
+// shares it, so calls that overlap on one GPU race; each call overwrites
+// every word it reads, so nothing needs clearing between calls.
+constexpr int kLngnMaxPartials = 65536;  // lattices of up to 65536 * 256 = 16,777,216 interior sites
+static __device__ __align__(256) double scratch[kLngnMaxPartials + 1];
+
+// the device address of `scratch` on each GPU (the host-side symbol is only a
+// stand-in), looked up once per device with that device current
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static double* scratch_on_current_device() {
+  int dev = 0;
+  SFX_HIP(hipGetDevice(&dev));
+  SFX_REQUIRE(dev >= 0 && dev < kMaxDevices, SFX_E_UNSUPPORTED, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    SFX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    SFX_REQUIRE(ptr != nullptr, SFX_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
+// total of the partial sums in index order (the order of k_sum_partials),
+// written to tot; with lngn != null also ln(total) there
+__global__ void k_lngn_finish(const double* __restrict__ partial, int n, double* __restrict__ tot,
+                              double* __restrict__ lngn) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0;
+    for (int i = 0; i < n; i++) s += partial[i];
+    *tot = s;
+    if (lngn) *lngn = log(s);
+  }
+}
 
 // SetBoundaries (Lat1DFlat.cpp:396-399, Lat2DFlat.cpp SetBoundaries,
 // Lat3D.cpp:1510-1562): halo sites copy their remapped source, surface halos
@@ -107,6 +148,28 @@ int sfx_lattice_lngn(const sfx_lattice* l, const double* G, double* lngn, void* 
     (void)hipFree(partial);
     (void)hipFree(out);
     *lngn = std::log(s);
+  });
+}
+
+int sfx_lattice_lngn_noalloc(const sfx_lattice* l, const double* G, double* lngn, int where, void* stream) {
+  return guarded([&] {
+    SFX_REQUIRE(l && G && lngn, SFX_E_INVALID, "NULL argument");
+    SFX_REQUIRE(where == SFX_HOST || where == SFX_DEVICE, SFX_E_INVALID, "where must be SFX_HOST or SFX_DEVICE");
+    const DevLattice& L = l->l.d;
+    const unsigned nb = grid1(L.V, 256);
+    SFX_REQUIRE(nb <= (unsigned)kLngnMaxPartials, SFX_E_INVALID,
+                "lattice too large for the allocation-free ln G_N (at most 16,777,216 interior sites)");
+    double* ptr = scratch_on_current_device();
+    double* total = ptr + kLngnMaxPartials;
+    launch_wsum<double>(L, G, ptr, S(stream));
+    k_lngn_finish<<<1, 64, 0, S(stream)>>>(ptr, (int)nb, total, where == SFX_DEVICE ? lngn : nullptr);
+    SFX_HIP(hipGetLastError());
+    if (where == SFX_HOST) {
+      double s = 0;
+      SFX_HIP(hipMemcpyAsync(&s, total, sizeof(double), hipMemcpyDeviceToHost, S(stream)));
+      SFX_HIP(hipStreamSynchronize(S(stream)));
+      *lngn = std::log(s);
+    }
   });
 }
 

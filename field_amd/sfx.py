@@ -63,6 +63,7 @@ _SIGS = {
     "sfx_lattice_correct_double_count": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "sfx_lattice_lngn": (C.c_int, [C.c_void_p, C.c_void_p, PD, C.c_void_p]),
     "sfx_lattice_set_boundaries": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfx_lattice_lngn_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "sfx_lattice_sides": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 

@@ -312,6 +312,17 @@ int sfx_lattice_correct_double_count(const sfx_lattice* l, double* in, const dou
 int sfx_lattice_lngn(const sfx_lattice* l, const double* G, double* lngn, void* stream);
 /* Fills the halo layers of P (SetBoundaries). */
 int sfx_lattice_set_boundaries(const sfx_lattice* l, double* P, void* stream);
+/* sfx_lattice_lngn without allocating: no hipMalloc, hipFree or allocating
+ * library call. It has no counterpart in the reference; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid it). The partial sums live in a static device array of 512 KiB per
+ * GPU, so the lattice may hold at most 16,777,216 interior sites (larger ones:
+ * SFX_E_INVALID). where = SFX_HOST: *lngn is a host double, the call waits for
+ * the stream and gives the same bits as sfx_lattice_lngn. where = SFX_DEVICE:
+ * lngn points to device memory, ln G_N is written there by a kernel and the call
+ * returns without waiting. Calls that overlap on one GPU share the array and race. */
+int sfx_lattice_lngn_noalloc(const sfx_lattice* l, const double* G, double* lngn, int where,
+                             void* stream);
 /* Contact average <P> (SideFraction on every site / Lat3D::Sides). */
 int sfx_lattice_sides(const sfx_lattice* l, const double* P, double* out, void* stream);
 
