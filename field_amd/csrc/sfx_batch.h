@@ -30,7 +30,7 @@ struct Batch {
   int B = 0;
   int n_max = 0;
   int block = 0;          // threads per problem, chosen by tune()
-  float tuned_ms[3] = {0, 0, 0};  // residual time at 64, 128, 256 threads
+  float tuned_ms[4] = {0, 0, 0, 0};  // residual time at 64, 128, 256, 512 threads
   long long nvar = 0, store_words = 0, phi_words = 0;
   std::vector<BatchProb> probs;
   BatchProb* d_prob = nullptr;
@@ -41,16 +41,8 @@ struct Batch {
   double* d_phi = nullptr;
   double* d_lngn = nullptr;
   BatchState* d_state = nullptr;
-  int* d_active = nullptr;
-  int* d_count = nullptr;
   int hist_m = 0;
   hipStream_t stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  int graph_key[3] = {0, 0, 0};  // block, diis_m, iterations per replay of the captured graph
-  double graph_tol = -1, graph_eta = 0;
-  int graph_max_iter = -1;
-  int graph_backtrack = -1;
   // results of the last solve
   std::vector<BatchState> last;
   int residual_launches = 0;
@@ -64,8 +56,7 @@ struct Batch {
  private:
   void ensure_history(int m);
   void launch_residual(int bt, const int* active);
-  void launch_diis(int bt, int m, double eta, double tol, int max_iter, int backtrack);
-  void drop_graph();
+  void launch_solve(int bt, int m, double eta, double tol, int max_iter, int backtrack);
 };
 
 }  // namespace sfx

@@ -9,9 +9,9 @@
 // homopolymer sweep (SF_Homopol1stO.cpp:552-622) with the running rank in LDS
 // and the stored ranks in HBM, the solvent closure (SF_MolList1stO.cpp:280-292),
 // the contact averages (Lat1DFlat.cpp:510-524) and the residual
-// (SF_Solve.cpp:554-590). A second kernel applies one DIIS step
-// (MPS.cpp:228-284) per problem. A captured graph replays several iterations
-// of both kernels between host checks.
+// (SF_Solve.cpp:554-590), then one DIIS step (MPS.cpp:228-284); a solve is
+// one launch in which every workgroup loops over its own problem's iterations
+// until that problem is done.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -35,20 +35,16 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return r;
 }
 
-// Residual of every active problem. LDS holds two ping-pong rank buffers of
+// The residual of problem b at x, by the workgroup that owns it; every
+// thread of the workgroup calls it. LDS holds two ping-pong rank buffers of
 // ldsw doubles; index 0 of both stays 0 (the frozen surface layer), so the
 // lower neighbour needs no branch; the mirror1 neighbour of layer n is n.
 template <int BT>
-__global__ void __launch_bounds__(BT) k_batch_residual(const BatchProb* __restrict__ P, const double* __restrict__ x,
-                                                       double* __restrict__ f, double* __restrict__ store,
-                                                       double* __restrict__ phi_out, double* __restrict__ lngn,
-                                                       const int* __restrict__ active, int ldsw) {
+__device__ void batch_residual(const BatchProb& p, int b, const double* __restrict__ x, double* __restrict__ f,
+                               double* __restrict__ store, double* __restrict__ phi_out,
+                               double* __restrict__ lngn, double* lds, int ldsw) {
   constexpr int K = kMaxLayers / BT;  // layers per thread
-  extern __shared__ double lds[];
   __shared__ double red[BT / 64];
-  const int b = blockIdx.x;
-  if (active && !active[b]) return;
-  const BatchProb p = P[b];
   const int n = p.n, N = p.N;
   const long long M = n + 2;
   const double l = p.lambda;
@@ -198,6 +194,38 @@ __global__ void __launch_bounds__(BT) k_batch_residual(const BatchProb* __restri
   }
 }
 
+template <int BT>
+__global__ void __launch_bounds__(BT) k_batch_residual(const BatchProb* __restrict__ P, const double* __restrict__ x,
+                                                       double* __restrict__ f, double* __restrict__ store,
+                                                       double* __restrict__ phi_out, double* __restrict__ lngn,
+                                                       const int* __restrict__ active, int ldsw) {
+  extern __shared__ double lds[];
+  const int b = blockIdx.x;
+  if (active && !active[b]) return;
+  const BatchProb p = P[b];
+  batch_residual<BT>(p, b, x, f, store, phi_out, lngn, lds, ldsw);
+}
+
+// Sums four values over the workgroup in a fixed order; every thread gets
+// the totals. red holds 4 * (BT / 64) doubles.
+template <int BT>
+__device__ __forceinline__ void block_reduce4(double (&v)[4], double* red) {
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    for (int o = 32; o > 0; o >>= 1) v[j] += __shfl_xor(v[j], o, 64);
+  __syncthreads();  // red may still be read from the previous call
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int j = 0; j < 4; j++) red[j * (BT / 64) + (threadIdx.x >> 6)] = v[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    double r = 0;
+    for (int w = 0; w < BT / 64; w++) r += red[j * (BT / 64) + w];
+    v[j] = r;
+  }
+}
+
 // X = A^{-1} 1 by Gaussian elimination with partial pivoting; A (k x k, row
 // major), rhs and X in LDS, one thread.
 __device__ void diis_solve(double* A, double* rhs, int k, double* X) {
@@ -239,17 +267,16 @@ __device__ void diis_solve(double* A, double* rhs, int k, double* X) {
 // stops there.
 constexpr double kGrow = 100.0;
 constexpr double kEtaFloor = 1e-4;
+// One DIIS step of problem b by the workgroup that owns it; returns true
+// (in every thread) when the problem is done. Every thread calls it.
 template <int BT>
-__global__ void __launch_bounds__(BT) k_batch_diis(const BatchProb* __restrict__ P, double* __restrict__ x,
-                                                   const double* __restrict__ g, double* __restrict__ hist,
-                                                   BatchState* __restrict__ S, int m, double eta, double tol,
-                                                   int max_iter, int backtrack, int* __restrict__ active) {
-  const int b = blockIdx.x;
-  if (!active[b]) return;
-  __shared__ double red[(kMaxDiis + 1) * (BT / 64)];
+__device__ bool batch_diis(const BatchProb& p, int b, double* __restrict__ x, const double* __restrict__ g,
+                           double* __restrict__ hist, BatchState* __restrict__ S, int m, double eta, double tol,
+                           int max_iter, int backtrack) {
+  __shared__ double red[4 * (BT / 64)];
+  __shared__ double dts[kMaxDiis + 1];
   __shared__ double Ap[kMaxDiis * kMaxDiis], rhs[kMaxDiis], coef[kMaxDiis];
-  __shared__ int verdict, save_best;
-  const BatchProb p = P[b];
+  __shared__ int verdict, save_best, finished;
   const long long nv = 2LL * p.n;
   double* xb = x + p.xoff;
   const double* gb = g + p.xoff;
@@ -266,10 +293,8 @@ __global__ void __launch_bounds__(BT) k_batch_diis(const BatchProb* __restrict__
   const int t = threadIdx.x;
   const double eta_p = backtrack ? st.eta : eta;
 
-  // new point x - eta g, its step, and the dot products of the step with the kept steps
-  double part[kMaxDiis + 1];
-#pragma unroll
-  for (int i = 0; i <= kMaxDiis; i++) part[i] = 0.0;
+  // new point x - eta g and its step; |g|^2 and the step's own product
+  double gg = 0.0, dd = 0.0;
   for (long long q = t; q < nv; q += BT) {
     const double xi = xb[q], gi = gb[q];
     const double xn = xi - eta_p * gi;
@@ -277,35 +302,39 @@ __global__ void __launch_bounds__(BT) k_batch_diis(const BatchProb* __restrict__
     x0[q] = xi;
     xR[(long long)k * nv + q] = xn;
     xx0[(long long)k * nv + q] = d;
-    part[kMaxDiis] += gi * gi;
-#pragma unroll
-    for (int i = 0; i < kMaxDiis; i++) {
-      if (i < kd - 1) {
-        int posi = k - kd + 1 + i;
-        if (posi < 0) posi += m;
-        part[i] += xx0[(long long)posi * nv + q] * d;
-      } else if (i == kd - 1) {
-        part[i] += d * d;
-      }
+    gg += gi * gi;
+    dd += d * d;
+  }
+  {
+    double two[4] = {gg, dd, 0.0, 0.0};
+    block_reduce4<BT>(two, red);
+    if (t == 0) {
+      dts[kMaxDiis] = two[0];
+      dts[kd - 1] = two[1];
     }
   }
-  // reduce the kd dots and |g|^2 over the block
+  // the products of the step with the kept steps, four per pass (a thread
+  // reads back only the step entries it wrote itself)
+  for (int i0 = 0; i0 < kd - 1; i0 += 4) {
+    double a4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long long q = t; q < nv; q += BT) {
+      const double d = xx0[(long long)k * nv + q];
 #pragma unroll
-  for (int i = 0; i <= kMaxDiis; i++)
-    for (int o = 32; o > 0; o >>= 1) part[i] += __shfl_xor(part[i], o, 64);
-  if ((t & 63) == 0) {
-#pragma unroll
-    for (int i = 0; i <= kMaxDiis; i++) red[i * (BT / 64) + (t >> 6)] = part[i];
+      for (int j = 0; j < 4; j++) {
+        if (i0 + j < kd - 1) {
+          int posi = k - kd + 1 + i0 + j;
+          if (posi < 0) posi += m;
+          a4[j] += xx0[(long long)posi * nv + q] * d;
+        }
+      }
+    }
+    block_reduce4<BT>(a4, red);
+    if (t == 0)
+      for (int j = 0; j < 4 && i0 + j < kd - 1; j++) dts[i0 + j] = a4[j];
   }
   __syncthreads();
   if (t == 0) {
-    double dots[kMaxDiis + 1];
-#pragma unroll
-    for (int i = 0; i <= kMaxDiis; i++) {
-      double r = 0;
-      for (int w = 0; w < BT / 64; w++) r += red[i * (BT / 64) + w];
-      dots[i] = r;
-    }
+    const double* dots = dts;
     const double err = sqrt(dots[kMaxDiis]);
     BatchState ns = st;
     ns.err = err;
@@ -356,66 +385,69 @@ __global__ void __launch_bounds__(BT) k_batch_diis(const BatchProb* __restrict__
       ns.k_diis = kd;
       if (it >= max_iter) ns.done = 2;
     }
-    if (ns.done) active[b] = 0;
     S[b] = ns;
     verdict = v;
     save_best = save;
+    finished = ns.done != 0;
   }
   __syncthreads();
+  const int vd = verdict;
+  const bool done = finished != 0;
   if (save_best)
     for (long long q = t; q < nv; q += BT) xbest[q] = x0[q];  // x0 holds the evaluated point
-  if (verdict == 1) return;  // x stays the point whose residual was measured
-  if (verdict == 3) {
+  if (vd == 3) {
     for (long long q = t; q < nv; q += BT) xb[q] = xbest[q];
-    return;
-  }
-  if (verdict == 2) {
+  } else if (vd == 2) {
     for (long long q = t; q < nv; q += BT) xb[q] = xR[(long long)k * nv + q];
-    return;
-  }
-  for (long long q = t; q < nv; q += BT) {
-    double s = 0;
-    for (int i = 0; i < kd; i++) {
-      int posi = k - kd + 1 + i;
-      if (posi < 0) posi += m;
-      s += coef[i] * xR[(long long)posi * nv + q];
+  } else if (vd == 0) {
+    for (long long q = t; q < nv; q += BT) {
+      double s = 0;
+      for (int i = 0; i < kd; i++) {
+        int posi = k - kd + 1 + i;
+        if (posi < 0) posi += m;
+        s += coef[i] * xR[(long long)posi * nv + q];
+      }
+      xb[q] = s;
     }
-    xb[q] = s;
-  }
-}
-
-__global__ void k_batch_count(const int* __restrict__ active, int n, int* __restrict__ count) {
-  __shared__ int red[4];
-  int c = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) c += active[i];
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  }  // vd == 1: x stays the point whose residual was measured
+  // the shared verdict words are rewritten by the next step's thread 0 only
+  // after every thread has read them here
   __syncthreads();
-  if (threadIdx.x == 0) *count = red[0] + red[1] + red[2] + red[3];
+  return done;
 }
 
-__global__ void k_batch_reset(int* __restrict__ active, BatchState* __restrict__ S, int n, double eta) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    active[i] = 1;
-    S[i] = BatchState{-1, 0, 0, 0, 0.0, eta, INFINITY};
+// The whole solve in one launch: each workgroup owns one problem and loops
+// residual -> DIIS step until that problem is done (converged, at the
+// iteration limit, non-finite or given up). Problems are independent, so no
+// workgroup ever waits on another and the launch ends when the slowest
+// problem does; the host neither synchronises nor relaunches per iteration.
+template <int BT>
+__global__ void __launch_bounds__(BT) k_batch_solve(const BatchProb* __restrict__ P, double* __restrict__ x,
+                                                    double* __restrict__ f, double* __restrict__ store,
+                                                    double* __restrict__ phi_out, double* __restrict__ lngn,
+                                                    double* __restrict__ hist, BatchState* __restrict__ S, int m,
+                                                    double eta, double tol, int max_iter, int backtrack, int ldsw) {
+  extern __shared__ double lds[];
+  const int b = blockIdx.x;
+  const BatchProb p = P[b];
+  while (true) {
+    batch_residual<BT>(p, b, x, f, store, phi_out, lngn, lds, ldsw);
+    __syncthreads();  // f and the rank buffers are read by other threads below
+    if (batch_diis<BT>(p, b, x, f, hist, S, m, eta, tol, max_iter, backtrack)) break;
   }
+}
+
+__global__ void k_batch_reset(BatchState* __restrict__ S, int n, double eta) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) S[i] = BatchState{-1, 0, 0, 0, 0.0, eta, INFINITY};
 }
 
 // ---------------------------------------------------------------------------
 Batch::~Batch() {
-  drop_graph();
-  void* ptrs[] = {d_prob, d_x, d_f, d_store, d_hist, d_phi, d_lngn, d_state, d_active, d_count};
+  void* ptrs[] = {d_prob, d_x, d_f, d_store, d_hist, d_phi, d_lngn, d_state};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (stream) (void)hipStreamDestroy(stream);
-}
-
-void Batch::drop_graph() {
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-  if (graph) (void)hipGraphDestroy(graph);
-  graph_exec = nullptr;
-  graph = nullptr;
 }
 
 void Batch::create(const sfx_adsorption_desc* d, int count, const sfx_options* opt) {
@@ -461,8 +493,6 @@ void Batch::create(const sfx_adsorption_desc* d, int count, const sfx_options* o
   SFX_HIP(hipMalloc(&d_phi, phi_words * sizeof(double)));
   SFX_HIP(hipMalloc(&d_lngn, B * sizeof(double)));
   SFX_HIP(hipMalloc(&d_state, B * sizeof(BatchState)));
-  SFX_HIP(hipMalloc(&d_active, B * sizeof(int)));
-  SFX_HIP(hipMalloc(&d_count, sizeof(int)));
   SFX_HIP(hipMemcpy(d_prob, probs.data(), B * sizeof(BatchProb), hipMemcpyHostToDevice));
   SFX_HIP(hipMemset(d_x, 0, nvar * sizeof(double)));
   tune();
@@ -478,40 +508,43 @@ void Batch::launch_residual(int bt, const int* active) {
     case 128:
       k_batch_residual<128><<<B, 128, shm, stream>>>(d_prob, d_x, d_f, d_store, d_phi, d_lngn, active, ldsw);
       break;
-    default:
+    case 256:
       k_batch_residual<256><<<B, 256, shm, stream>>>(d_prob, d_x, d_f, d_store, d_phi, d_lngn, active, ldsw);
       break;
-  }
-  SFX_HIP(hipGetLastError());
-}
-
-void Batch::launch_diis(int bt, int m, double eta, double tol, int max_iter, int backtrack) {
-  switch (bt) {
-    case 64:
-      k_batch_diis<64><<<B, 64, 0, stream>>>(d_prob, d_x, d_f, d_hist, d_state, m, eta, tol, max_iter, backtrack,
-                                             d_active);
-      break;
-    case 128:
-      k_batch_diis<128><<<B, 128, 0, stream>>>(d_prob, d_x, d_f, d_hist, d_state, m, eta, tol, max_iter, backtrack,
-                                               d_active);
-      break;
     default:
-      k_batch_diis<256><<<B, 256, 0, stream>>>(d_prob, d_x, d_f, d_hist, d_state, m, eta, tol, max_iter, backtrack,
-                                               d_active);
+      k_batch_residual<512><<<B, 512, shm, stream>>>(d_prob, d_x, d_f, d_store, d_phi, d_lngn, active, ldsw);
       break;
   }
   SFX_HIP(hipGetLastError());
 }
 
-// Threads per problem: time one residual of the whole batch at 64, 128 and
-// 256 threads (at the current x) and keep the fastest.
+void Batch::launch_solve(int bt, int m, double eta, double tol, int max_iter, int backtrack) {
+  const int ldsw = n_max + 2;
+  const size_t shm = 2 * (size_t)ldsw * sizeof(double);
+#define SOLVE(BT)                                                                                              \
+  k_batch_solve<BT><<<B, BT, shm, stream>>>(d_prob, d_x, d_f, d_store, d_phi, d_lngn, d_hist, d_state, m, eta, \
+                                            tol, max_iter, backtrack, ldsw)
+  switch (bt) {
+    case 64: SOLVE(64); break;
+    case 128: SOLVE(128); break;
+    case 256: SOLVE(256); break;
+    default: SOLVE(512); break;
+  }
+#undef SOLVE
+  SFX_HIP(hipGetLastError());
+}
+
+// Threads per problem: time one residual of the whole batch at 64, 128, 256
+// and 512 threads (at the current x) and keep the fastest. A solve iteration
+// is one residual plus a DIIS step of a few vector passes, so the residual
+// decides.
 void Batch::tune() {
   hipEvent_t e0, e1;
   SFX_HIP(hipEventCreate(&e0));
   SFX_HIP(hipEventCreate(&e1));
   float best = 1e30f;
   block = 256;
-  for (int bt : {64, 128, 256}) {
+  for (int bt : {64, 128, 256, 512}) {
     launch_residual(bt, nullptr);  // warm-up
     SFX_HIP(hipEventRecord(e0, stream));
     for (int r = 0; r < 3; r++) launch_residual(bt, nullptr);
@@ -519,7 +552,7 @@ void Batch::tune() {
     SFX_HIP(hipEventSynchronize(e1));
     float ms = 0;
     SFX_HIP(hipEventElapsedTime(&ms, e0, e1));
-    tuned_ms[bt == 64 ? 0 : bt == 128 ? 1 : 2] = ms / 3;
+    tuned_ms[bt == 64 ? 0 : bt == 128 ? 1 : bt == 256 ? 2 : 3] = ms / 3;
     if (ms < best) {
       best = ms;
       block = bt;
@@ -531,7 +564,6 @@ void Batch::tune() {
 
 void Batch::ensure_history(int m) {
   if (hist_m == m) return;
-  drop_graph();
   if (d_hist) SFX_HIP(hipFree(d_hist));
   d_hist = nullptr;
   long long ho = 0;
@@ -562,45 +594,14 @@ void Batch::solve(const sfx_solver_opts& o, double* x, int where, int* iters, do
   const bool host = where == SFX_HOST;
   SFX_HIP(hipMemcpyAsync(d_x, x, nvar * sizeof(double), host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
                          stream));
-  k_batch_reset<<<grid1(B, 256), 256, 0, stream>>>(d_active, d_state, B, o.diis_eta);
+  k_batch_reset<<<grid1(B, 256), 256, 0, stream>>>(d_state, B, o.diis_eta);
   SFX_HIP(hipGetLastError());
-
-  // the captured graph: R iterations (residual + DIIS step) and the count of active problems
-  const int R = 8;
-  const bool same = graph_exec && graph_key[0] == block && graph_key[1] == o.diis_m && graph_key[2] == R &&
-                    graph_tol == o.tolerance && graph_eta == o.diis_eta && graph_max_iter == o.max_iter &&
-                    graph_backtrack == o.diis_backtrack;
-  if (!same) {
-    drop_graph();
-    SFX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (int r = 0; r < R; r++) {
-      launch_residual(block, d_active);
-      launch_diis(block, o.diis_m, o.diis_eta, o.tolerance, o.max_iter, o.diis_backtrack);
-    }
-    k_batch_count<<<1, 256, 0, stream>>>(d_active, B, d_count);
-    SFX_HIP(hipGetLastError());
-    SFX_HIP(hipStreamEndCapture(stream, &graph));
-    SFX_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-    graph_key[0] = block;
-    graph_key[1] = o.diis_m;
-    graph_key[2] = R;
-    graph_tol = o.tolerance;
-    graph_eta = o.diis_eta;
-    graph_max_iter = o.max_iter;
-    graph_backtrack = o.diis_backtrack;
-  }
   hipEvent_t e0, e1;
   SFX_HIP(hipEventCreate(&e0));
   SFX_HIP(hipEventCreate(&e1));
   SFX_HIP(hipEventRecord(e0, stream));
-  // every problem stops after at most max_iter + 1 evaluations
-  const int replays = (o.max_iter + 1 + R - 1) / R;
-  int left = B;
-  for (int r = 0; r < replays && left > 0; r++) {
-    SFX_HIP(hipGraphLaunch(graph_exec, stream));
-    SFX_HIP(hipMemcpyAsync(&left, d_count, sizeof(int), hipMemcpyDeviceToHost, stream));
-    SFX_HIP(hipStreamSynchronize(stream));
-  }
+  // one launch: every problem stops after at most max_iter + 1 evaluations
+  launch_solve(block, o.diis_m, o.diis_eta, o.tolerance, o.max_iter, o.diis_backtrack);
   SFX_HIP(hipEventRecord(e1, stream));
   SFX_HIP(hipEventSynchronize(e1));
   float ms = 0;
@@ -608,7 +609,7 @@ void Batch::solve(const sfx_solver_opts& o, double* x, int where, int* iters, do
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   last.resize(B);
-  SFX_HIP(hipMemcpy(last.data(), d_state, B * sizeof(BatchState), hipMemcpyDeviceToHost));
+  SFX_HIP(hipMemcpyAsync(last.data(), d_state, B * sizeof(BatchState), hipMemcpyDeviceToHost, stream));
   SFX_HIP(hipMemcpyAsync(x, d_x, nvar * sizeof(double), host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
                          stream));
   SFX_HIP(hipStreamSynchronize(stream));
@@ -672,7 +673,7 @@ int sfx_batch_tuning(const sfx_batch* b, int* threads, float* ms) {
     SFX_REQUIRE(b && threads, SFX_E_INVALID, "NULL argument");
     *threads = b->b.block;
     if (ms)
-      for (int i = 0; i < 3; i++) ms[i] = b->b.tuned_ms[i];
+      for (int i = 0; i < 4; i++) ms[i] = b->b.tuned_ms[i];
   });
 }
 

@@ -351,7 +351,7 @@ class Batch:
 
     def tuning(self):
         th = C.c_int()
-        ms = (C.c_float * 3)()
+        ms = (C.c_float * 4)()
         check(lib().sfx_batch_tuning(self.h, C.byref(th), ms))
         return th.value, list(ms)
 

@@ -276,11 +276,13 @@ int sfx_batch_destroy(sfx_batch* b);
  * problem's part of the iteration vector. */
 int sfx_batch_sizes(const sfx_batch* b, long long* nvar, long long* offsets);
 /* Threads per problem the tuner chose at creation and the residual time (ms)
- * it measured at 64, 128 and 256 threads (ms may be NULL). */
+ * it measured at 64, 128, 256 and 512 threads (ms: 4 floats, may be NULL). */
 int sfx_batch_tuning(const sfx_batch* b, int* threads, float* ms);
 /* Residuals of every problem at x (host or device per `where`). */
 int sfx_batch_residual(sfx_batch* b, const double* x, double* f, int where);
-/* DIIS (MPS.cpp:228-284) on every problem at once from x (in/out). Per
+/* DIIS (MPS.cpp:228-284) on every problem at once from x (in/out), in one
+ * kernel launch: each problem's workgroup iterates until that problem is done,
+ * with no host synchronisation in between. Per
  * problem: evaluations after the first (iterations) and the final residual
  * norm (host arrays of count, may be NULL). A converged problem returns the
  * point whose residual was measured. SFX_E_NOTCONVERGED if any problem did

@@ -39,7 +39,7 @@ def test_batch_layout_and_tuner(batch):
     assert list(np.diff(batch.offsets)) == [2 * p["n_layers"] for p in PROBLEMS]
     threads, ms = batch.tuning()
     print("tuner:", threads, ms)
-    assert threads in (64, 128, 256) and all(m > 0 for m in ms)
+    assert threads in (64, 128, 256, 512) and all(m > 0 for m in ms)
 
 
 def test_batch_residual_matches_oracle(batch, orc):
@@ -146,3 +146,30 @@ def test_uneven_sweep_backtracking(sfx, orc):
         lat, sysd, _ = configs.adsorption_problem(probs[i])
         O = orc.System(lat, sysd)
         assert np.linalg.norm(O.residual(B.split(x)[i])) <= 1e-7 * (1 + 1e-6)
+
+
+def test_uneven_sweep_full_size(sfx, orc):
+    """The whole 512-problem sweep (layers 50..400, N 20..1000, chi_AS 0..1,
+    chi_s -2..0, phi_b 1e-4..1e-1) in one persistent launch with per-problem
+    backtracking: no problem ends non-finite, every problem whose bulk is a
+    stable Flory-Huggins solution converges, and every problem reported
+    converged has an oracle residual below the tolerance at the returned
+    point."""
+    probs = configs.batch_1d(512)
+    B = sfx.Batch(probs)
+    print("512 problems, tuner", B.tuning(), flush=True)
+    x, its, norms, st = B.solve(tolerance=1e-7, max_iter=5000, diis_backtrack=1, diis_eta=1.0)
+    conv = norms <= 1e-7
+    stable = np.array([configs.bulk_is_stable(p) for p in probs])
+    print(f"512 problems: {conv.sum()} converged, {conv[stable].sum()}/{stable.sum()} with a stable bulk, "
+          f"{st['seconds']:.3f} s, max iterations {its.max()}", flush=True)
+    assert np.isfinite(norms).all()
+    assert conv[stable].all()
+    xs = B.split(x)
+    worst = 0.0
+    for i in np.flatnonzero(conv):
+        lat, sysd, _ = configs.adsorption_problem(probs[i])
+        r = np.linalg.norm(orc.System(lat, sysd).residual(xs[i]))
+        worst = max(worst, r)
+        assert r <= 1e-7 * (1 + 1e-6), (i, probs[i], r)
+    print(f"largest oracle residual over the {conv.sum()} converged problems: {worst:.3e}")
