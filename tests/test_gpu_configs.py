@@ -1,7 +1,7 @@
 """The five workloads at their full sizes, on the GPU through the C ABI.
 
-Where the oracle finishes in seconds (the 1-D wall, the 2-D cylindrical pore)
-it checks the GPU directly at full size; for the 128^3 second-order system it
+Where the oracle finishes in seconds (the 64^3 melt, the 1-D wall, the 2-D
+cylindrical pore) it checks the GPU directly at full size; for the 128^3 second-order system it
 would take minutes, so that one is held to properties that do not depend on
 size: FP32 against FP64, sqrt(N) checkpointing against the full store, and
 the Jacobian product against central differences of the GPU's FP64
@@ -12,6 +12,49 @@ import pytest
 from field_amd import configs
 
 pytestmark = pytest.mark.gpu
+
+
+def test_melt64_against_oracle_at_the_seeded_field(sfx, orc):
+    """The 64^3 A12B28 melt (chi = 0.4, u ~ N(0, 0.1), seed 3) at its seeded
+    field, where the profile is far from homogeneous: FP64 residual, phi per
+    state, ln G_N and free energy against the oracle to 1e-10, and J v
+    against the oracle's own central differences (h = 1e-6, FP64) on 16
+    random directions; the FP32 path's errors against the FP64 oracle are
+    reported."""
+    lat, sysd, x = configs.melt_3d()
+    G, O = sfx.System(lat, sysd), orc.System(lat, sysd)
+    print("64^3 melt: FP64 residual, GPU and oracle", flush=True)
+    fg = G.residual(x, precision="fp64")
+    fo = O.residual(x)
+    np.testing.assert_allclose(fg, fo, rtol=1e-10, atol=1e-12)
+    assert np.abs(fo).max() > 1e-2  # far from a solution: the field is inhomogeneous
+    for t in range(len(sysd.states)):
+        np.testing.assert_allclose(G.phi(t), O.phi(t), rtol=1e-10, atol=1e-13)
+    assert abs(G.lngn(0) - O.lngn(0)) <= 1e-10 * abs(O.lngn(0))
+    Fg, Fo = G.free_energy(), O.free_energy()
+    assert abs(Fg - Fo) <= 1e-10 * abs(Fo), (Fg, Fo)
+    f32 = G.residual(x, precision="fp32")
+    F32 = G.free_energy()
+    print(f"FP32 path vs FP64 oracle: residual max abs err {np.abs(f32 - fo).max():.2e}, "
+          f"free energy rel err {abs(F32 - Fo) / abs(Fo):.2e}", flush=True)
+    assert np.abs(f32 - fo).max() < 2e-4
+    mask = np.tile(lat.interior_mask().reshape(-1), 2)
+    rng = np.random.default_rng(33)
+    h = 1e-6
+    worst, worst32 = 0.0, 0.0
+    for k in range(16):
+        v = rng.normal(0, 1, G.nvar) * mask
+        fd = (O.residual(x + h * v) - O.residual(x - h * v)) / (2 * h)
+        jv = G.jvp(x, v, precision="fp64")
+        j32 = G.jvp(x, v, precision="fp32")
+        scale = np.abs(fd).max()
+        worst = max(worst, np.abs(jv - fd).max() / scale)
+        worst32 = max(worst32, np.abs(j32 - fd).max() / scale)
+        print(f"direction {k}: FP64 J v vs oracle central differences {np.abs(jv - fd).max() / scale:.2e}",
+              flush=True)
+    print(f"64^3 melt: worst FP64 {worst:.2e}, FP32 {worst32:.2e}")
+    assert worst < 1e-6
+    assert worst32 < 1e-3
 
 
 def test_wall_adsorption_full_size(sfx, orc):

@@ -67,3 +67,60 @@ def test_gibbs_adsorption_equation(orc):
     print(f"dOmega = {lhs:.6e}, -sum Gamma dmu = {rhs:.6e}")
     assert abs(lhs) > 1e-7
     assert abs(lhs - rhs) < 1e-5 * abs(lhs)
+
+
+def _site_volumes(orc, lat):
+    """Volume of every interior site: the weight ComputeLnGN gives it, read
+    as exp(ln G_N) of a unit vector on that site (layer volumes L(z) on curved
+    lattices, 1/2 on a mirror2 boundary layer, 1 elsewhere)."""
+    L = orc.Lattice(lat)
+    w = np.zeros(lat.M)
+    for i in np.flatnonzero(lat.interior_mask().reshape(-1)):
+        e = np.zeros(lat.M)
+        e[i] = 1.0
+        w[i] = math.exp(L.lngn(e))
+    return w
+
+
+def _exclusive_sf_cases():
+    import cases
+    out = []
+    for name in cases.NAMES:
+        lat, sysd = cases.case(name)
+        if sysd.form != "sfbox":
+            continue  # the melt form has no grand potential
+        owner = {}
+        shared = False
+        for m, seq in enumerate(sysd.sequences()):
+            for t in set(seq):
+                shared |= t in owner
+                owner[t] = m
+        if not shared:  # each state belongs to one molecule, so its phi is that molecule's
+            out.append(name)
+    return out
+
+
+@pytest.mark.parametrize("name", _exclusive_sf_cases())
+def test_helmholtz_minus_grand_potential_is_sum_mu_n(orc, name):
+    """At a converged profile F - Omega = sum_m mu_m n_m, n_m the number of
+    molecules m (sum over sites of volume x phi_m, divided by N_m). The
+    identity holds only where the residual vanishes, and it ties the free
+    energy to the grand and chemical potentials pinned above on walls, curved
+    lattices, pores, restricted chains and second order alike: a wrong
+    boundary subtraction or layer weight in F breaks it."""
+    import cases
+    lat, sysd = cases.case(name)
+    O = orc.System(lat, sysd)
+    x, its, err, rc = O.solve_diis(cases.start(lat, sysd, seed=3, sigma=0.01), tol=1e-11, max_iter=30000)
+    assert rc == 0 and err < 1e-11
+    O.residual(x)
+    w = _site_volumes(orc, lat)
+    F, omega = O.free_energy(), O.grand_potential()
+    mu_n = 0.0
+    for m, seq in enumerate(sysd.sequences()):
+        phi_m = sum(O.phi(t) for t in set(seq))
+        mu_n += O.chemical_potential(m) * np.sum(w * phi_m) / len(seq)
+    print(f"{name}: F = {F:.10f}, Omega = {omega:.10f}, F - Omega - sum mu n = {F - omega - mu_n:.3e}")
+    assert abs(F - omega - mu_n) <= 1e-8 * max(1.0, abs(F))
+    # not trivially so: the profile is inhomogeneous and F differs from Omega
+    assert abs(F - omega) > 1e-3
