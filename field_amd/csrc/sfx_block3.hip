@@ -12,10 +12,12 @@
 // Every thread owns one tile site: it stores that site's value of every rank,
 // composes acc += G_k G_pair c_k (ConnectG, Lat3D1stO.cpp:299-312; tangent
 // dacc += (dG_k G_pair + G_k dG_pair) c_k) in a register with the partner
-// values prefetched at launch start, and adds it to HBM once. The widened
-// ranks are computed by (y, z) columns, several threads per column along x;
-// consecutive threads take consecutive LDS words (whole rows), so LDS reads are
-// conflict-free, and HBM accesses coalesce along z.
+// values prefetched at launch start, and adds it to HBM once. In FP32 the
+// widened ranks are computed four z sites per work item with float4 LDS
+// accesses (consecutive items take consecutive 16-byte words of whole rows);
+// in FP64 by (y, z) columns, several threads per column along x, consecutive
+// threads on consecutive LDS words. LDS reads are conflict-free either way,
+// and HBM accesses coalesce along z.
 //
 // FP32 scaling: the input is divided by max(rank s0) and every rank by the
 // lattice maximum of g, so no rank exceeds 1 and at most nr ranks decay
@@ -153,11 +155,12 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Launch<T> la, in
   // scale factors of this launch. They are uniform, so every wave evaluates
   // each FP64 log / exp once in one lane and broadcasts it: lanes 0 and 1 the
   // logs of the two divisors, lane k the composition factor of rank k + 1.
+  // The FP64 logs and exponentials are evaluated by the first wave alone,
+  // which shares the composition factors through LDS (read after the first
+  // rank's barrier); the other waves go on to the LDS stores meanwhile.
   T in_scale = T(1), ginv = T(1);
   double ln_base = 0.0, lng = 0.0;
-  T cfac[kB3Ranks];
-#pragma unroll
-  for (int k = 0; k < kB3Ranks; k++) cfac[k] = T(1);
+  __shared__ T cfac[kB3Ranks];
   if constexpr (SCALED) {
     float m0, gm;
     block_step_max2(a.maxv, a.s0, a.gmax, 0, m0, gm);
@@ -165,14 +168,17 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Launch<T> la, in
     if (!(gm > 0.f)) gm = 1.f;
     in_scale = (T)(1.f / m0);
     ginv = (T)(1.f / gm);
-    const double lv = lane == 0 ? log((double)m0) : (lane == 1 ? log((double)gm) : 0.0);
-    ln_base = lnv0 + __shfl(lv, 0, 64);
-    lng = __shfl(lv, 1, 64);
-    if constexpr (CONNECT) {
-      const double c = lane < nr ? exp(ln_base + (lane + 1) * lng + lnp - lref) : 1.0;
-#pragma unroll
-      for (int k = 0; k < kB3Ranks; k++) cfac[k] = (T)__shfl(c, k, 64);
+    if (t < 64) {
+      const double lv = lane == 0 ? log((double)m0) : (lane == 1 ? log((double)gm) : 0.0);
+      ln_base = lnv0 + __shfl(lv, 0, 64);
+      lng = __shfl(lv, 1, 64);
+      if constexpr (CONNECT) {
+        const double c = lane < nr ? exp(ln_base + (lane + 1) * lng + lnp - lref) : 1.0;
+        if (lane < kB3Ranks) cfac[lane] = (T)c;
+      }
     }
+  } else if constexpr (CONNECT) {
+    if (t < kB3Ranks) cfac[t] = T(1);
   }
 
 #pragma unroll
@@ -202,21 +208,61 @@ __global__ void __launch_bounds__(kB3) k_block3(DevLattice L, B3Launch<T> la, in
     const int e = nr - k;
     const int lo = kH - e;
     const int bxn = kTX + 2 * e, byn = kTY + 2 * e, bzn = kTZ + 2 * e;
-    const int ncol = byn * kRZ;
-    const int per = kB3 / ncol > 0 ? kB3 / ncol : 1;
-    const int c = t % ncol, xs = t / ncol, RZ = c % kRZ;
-    if (t < per * ncol && RZ >= lo && RZ < lo + bzn) {
-      const int base = (lo + c / kRZ) * kRZ + RZ;
-      for (int X = lo + xs; X < lo + bxn; X += per) {
-        const int idx = X * kRYZ + base;
-        const T s = (src[idx - kRYZ] + src[idx + kRYZ] + src[idx - kRZ] + src[idx + kRZ] + src[idx - 1] +
-                     src[idx + 1]) * sixth;
-        const T gv = gl[idx] * ginv;
-        dst[idx] = gv * s;
+    if constexpr (sizeof(T) == 4) {
+      // FP32: a work item is four consecutive z sites of one (x, y) row, read
+      // and written as float4 (rows start on 16-byte boundaries: kRZ and kRYZ
+      // are multiples of 4). Items cover the widened z range rounded out to
+      // whole quads; the extra sites at the row ends read in-buffer
+      // neighbours, and no later rank reads them. Same additions in the same
+      // order as one site at a time, so the same bits.
+      static_assert(kRZ % 4 == 0 && kRYZ % 4 == 0, "float4 rows");
+      const int c0 = lo >> 2, nch = ((lo + bzn + 3) >> 2) - c0;
+      const int items = bxn * byn * nch;
+      for (int it = t; it < items; it += kB3) {
+        const int ch = c0 + it % nch, r = it / nch;
+        const int idx = (lo + r / byn) * kRYZ + (lo + r % byn) * kRZ + ch * 4;
+        const float4 c = *(const float4*)&src[idx];
+        const float4 xm = *(const float4*)&src[idx - kRYZ], xp = *(const float4*)&src[idx + kRYZ];
+        const float4 ym = *(const float4*)&src[idx - kRZ], yp = *(const float4*)&src[idx + kRZ];
+        const float zm = src[idx - 1], zp = src[idx + 4];
+        const float4 gq = *(const float4*)&gl[idx];
+        const float4 sv = make_float4((xm.x + xp.x + ym.x + yp.x + zm + c.y) * sixth,
+                                      (xm.y + xp.y + ym.y + yp.y + c.x + c.z) * sixth,
+                                      (xm.z + xp.z + ym.z + yp.z + c.y + c.w) * sixth,
+                                      (xm.w + xp.w + ym.w + yp.w + c.z + zp) * sixth);
+        const float4 gv = make_float4(gq.x * ginv, gq.y * ginv, gq.z * ginv, gq.w * ginv);
+        *(float4*)&dst[idx] = make_float4(gv.x * sv.x, gv.y * sv.y, gv.z * sv.z, gv.w * sv.w);
         if constexpr (TANGENT) {
-          const T ds = (dsrc[idx - kRYZ] + dsrc[idx + kRYZ] + dsrc[idx - kRZ] + dsrc[idx + kRZ] + dsrc[idx - 1] +
-                        dsrc[idx + 1]) * sixth;
-          ddst[idx] = gv * (ds - vl[idx] * s);
+          const float4 dc = *(const float4*)&dsrc[idx];
+          const float4 dxm = *(const float4*)&dsrc[idx - kRYZ], dxp = *(const float4*)&dsrc[idx + kRYZ];
+          const float4 dym = *(const float4*)&dsrc[idx - kRZ], dyp = *(const float4*)&dsrc[idx + kRZ];
+          const float dzm = dsrc[idx - 1], dzp = dsrc[idx + 4];
+          const float4 vq = *(const float4*)&vl[idx];
+          const float4 ds = make_float4((dxm.x + dxp.x + dym.x + dyp.x + dzm + dc.y) * sixth,
+                                        (dxm.y + dxp.y + dym.y + dyp.y + dc.x + dc.z) * sixth,
+                                        (dxm.z + dxp.z + dym.z + dyp.z + dc.y + dc.w) * sixth,
+                                        (dxm.w + dxp.w + dym.w + dyp.w + dc.z + dzp) * sixth);
+          *(float4*)&ddst[idx] = make_float4(gv.x * (ds.x - vq.x * sv.x), gv.y * (ds.y - vq.y * sv.y),
+                                             gv.z * (ds.z - vq.z * sv.z), gv.w * (ds.w - vq.w * sv.w));
+        }
+      }
+    } else {
+      const int ncol = byn * kRZ;
+      const int per = kB3 / ncol > 0 ? kB3 / ncol : 1;
+      const int c = t % ncol, xs = t / ncol, RZ = c % kRZ;
+      if (t < per * ncol && RZ >= lo && RZ < lo + bzn) {
+        const int base = (lo + c / kRZ) * kRZ + RZ;
+        for (int X = lo + xs; X < lo + bxn; X += per) {
+          const int idx = X * kRYZ + base;
+          const T s = (src[idx - kRYZ] + src[idx + kRYZ] + src[idx - kRZ] + src[idx + kRZ] + src[idx - 1] +
+                       src[idx + 1]) * sixth;
+          const T gv = gl[idx] * ginv;
+          dst[idx] = gv * s;
+          if constexpr (TANGENT) {
+            const T ds = (dsrc[idx - kRYZ] + dsrc[idx + kRYZ] + dsrc[idx - kRZ] + dsrc[idx + kRZ] + dsrc[idx - 1] +
+                          dsrc[idx + 1]) * sixth;
+            ddst[idx] = gv * (ds - vl[idx] * s);
+          }
         }
       }
     }
