@@ -113,4 +113,11 @@ inline unsigned grid1(long long n, int block) { return (unsigned)((n + block - 1
 // opaque handle of the per-rank lattice primitives (include/sfx.h)
 struct sfx_lattice {
   sfx::Lattice l;
+  // reduction partials and one result word of the normalisations, allocated
+  // on first use and kept (sfx_lattice_norm_phi_restr)
+  double* d_work = nullptr;
+  long long nwork = 0;
+  ~sfx_lattice() {
+    if (d_work) (void)hipFree(d_work);
+  }
 };

@@ -64,6 +64,16 @@ _SIGS = {
     "sfx_lattice_lngn": (C.c_int, [C.c_void_p, C.c_void_p, PD, C.c_void_p]),
     "sfx_lattice_set_boundaries": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "sfx_lattice_lngn_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "sfx_lattice_propagate_force": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                              C.c_void_p]),
+    "sfx_lattice_norm_phi_free": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
+    "sfx_lattice_norm_phi_restr": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                             C.c_void_p]),
+    "sfx_lattice_propagate2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
+                                         C.c_void_p]),
+    "sfx_lattice_connect2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sfx_lattice_correct_double_count2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                    C.c_void_p]),
     "sfx_lattice_sides": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 

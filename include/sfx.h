@@ -310,7 +310,9 @@ int sfx_lattice_connect(const sfx_lattice* l, const double* A, const double* B,
 /* in /= G where G > 0, else 0 (CorrectDoubleCountG). */
 int sfx_lattice_correct_double_count(const sfx_lattice* l, double* in, const double* G,
                                      void* stream);
-/* ln of the site-weighted sum of G over the interior (ComputeLnGN). */
+/* ln of the site-weighted sum of G over the interior (ComputeLnGN). On a 3-D
+ * second-order lattice G holds six direction components (6*M) and the result
+ * is ln(sum_k sum G^k / 6) (Lat3D2ndO::ComputeLnGN, Lat3D2ndO.cpp:996-1025). */
 int sfx_lattice_lngn(const sfx_lattice* l, const double* G, double* lngn, void* stream);
 /* Fills the halo layers of P (SetBoundaries). */
 int sfx_lattice_set_boundaries(const sfx_lattice* l, double* P, void* stream);
@@ -327,6 +329,46 @@ int sfx_lattice_lngn_noalloc(const sfx_lattice* l, const double* G, double* lngn
                              void* stream);
 /* Contact average <P> (SideFraction on every site / Lat3D::Sides). */
 int sfx_lattice_sides(const sfx_lattice* l, const double* P, double* out, void* stream);
+/* Gout = G * stencil_f(Gin) under a force f along the last gradient: a
+ * neighbour one layer below weighs e^f, one above e^-f, normalised as the
+ * reference does (Lat1stO::PropagateG(Gi,G,Gout,f): Lat1DFlat1stO.cpp:110-130,
+ * Lat3D1stO.cpp:94-165, Lat3DFCC1stO.cpp:102-145, Lat3DHEX1stO.cpp:102-145).
+ * Flat 1-D and 3-D lattices; 2-D and curved ones: SFX_E_UNSUPPORTED (the
+ * reference stops there: Lat2DFlat1stO.cpp:55-57). A surface neighbour
+ * contributes 0, as in sfx_lattice_propagate (a surface's propagator halo
+ * holds 0 in the reference's sweeps). Halos of Gout filled. */
+int sfx_lattice_propagate_force(const sfx_lattice* l, const double* Gin, const double* G, double* Gout,
+                                double f, void* stream);
+/* Normalisations (Lat1stO.h:19-50, Lat2ndO.h:62-66): phi *= C on every site
+ * (NormPhiFree, Lat1DFlat1stO.cpp:318-322), or phi *= C / exp(ComputeLnGN(Gi))
+ * (NormPhiRestr, :324-331); on a 3-D second-order lattice Gi holds six
+ * components, the divisor is 6 exp(ComputeLnGN(Gi)) and PHI (3*M, the per-axis
+ * parts, may be NULL) is scaled alike (Lat3D2ndO.cpp:1027-1047). The restricted
+ * form reduces Gi on the device: no host synchronisation. */
+int sfx_lattice_norm_phi_free(const sfx_lattice* l, double* phi, double* PHI, double C, void* stream);
+int sfx_lattice_norm_phi_restr(sfx_lattice* l, double* phi, double* PHI, const double* Gi, double C,
+                               void* stream);
+/* ---- second order, 3-D (Lat2ndO.h:19-66, Cuda_tools.h Forward/Backward
+ * with S): arrays of six direction components, component k of site i at
+ * k*M + i, k the bond direction {+x, +y, +z, -z, -y, -x}; S the straight-bond
+ * weight, (1-S)/4 per perpendicular bond, reversal 0. 1-D and 2-D
+ * second-order lattices (the reference's three and five lumped components):
+ * SFX_E_UNSUPPORTED. */
+/* backward = 0: PropagateF, Gout^k(r) = G(r) sum_j T_kj Gin^j(r - e_k)
+ * (Lat3D2ndO.cpp:64-152); backward = 1: PropagateB, Gout^k(r) =
+ * G(r) sum_j T_jk Gin^j(r + e_j) (:597-678). Input halos are read through the
+ * boundary maps (mirror sides reverse their axis' components), Gout's halos
+ * are filled. Gin and Gout must not overlap. */
+int sfx_lattice_propagate2(const sfx_lattice* l, const double* Gin, const double* G, double* Gout,
+                           double S, int backward, void* stream);
+/* out += sum_k A^k B^k on every site; OUT (3*M, may be NULL) += A^k B^k into
+ * the part of k's axis (ConnectG with PHI, Lat3D2ndO.cpp:895-955). */
+int sfx_lattice_connect2(const sfx_lattice* l, const double* A, const double* B, double* out,
+                         double* OUT, void* stream);
+/* in /= G where G > 0, else 0, and the same for the three parts of IN (3*M,
+ * may be NULL) (CorrectDoubleCountG with PHI, Lat3D2ndO.cpp:988-994). */
+int sfx_lattice_correct_double_count2(const sfx_lattice* l, double* in, double* IN, const double* G,
+                                      void* stream);
 
 #ifdef __cplusplus
 }

@@ -42,6 +42,12 @@ int orc_ncomp2(const orc_lat* L);
 double orc_wcomp2(const orc_lat* L, int k);
 void orc_propagate2(const orc_lat* L, double* Gin, const double* g, double* Gout, double S, int backward);
 double orc_lngn2(const orc_lat* L, const double* G);
+void orc_connect2(const orc_lat* L, const double* A, const double* B, double* out, double* OUT);
+void orc_correct2(const orc_lat* L, double* in, double* IN, const double* G);
+/* force ensemble and normalisations */
+void orc_propagate_force(const orc_lat* L, double* Gin, const double* g, double* Gout, double f);
+void orc_norm_phi_free(const orc_lat* L, double* phi, double* PHI, double C);
+void orc_norm_phi_restr(const orc_lat* L, double* phi, double* PHI, const double* Gi, double C);
 /* tests only: 1 makes the 2-D cylinder's backward sweep split its outward
  * radial step as the forward sweep does (the reference does not) */
 void orc_set_cyl2d_split_outward(int on);

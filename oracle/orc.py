@@ -80,6 +80,13 @@ def lib():
         L.orc_propagate2.argtypes = [C.c_void_p, PD, PD, PD, D, C.c_int]
         L.orc_ncomp2.argtypes = [C.c_void_p]
         L.orc_set_cyl2d_split_outward.argtypes = [C.c_int]
+        L.orc_lngn2.restype = D
+        L.orc_lngn2.argtypes = [C.c_void_p, PD]
+        L.orc_connect2.argtypes = [C.c_void_p, PD, PD, PD, PD]
+        L.orc_correct2.argtypes = [C.c_void_p, PD, PD, PD]
+        L.orc_propagate_force.argtypes = [C.c_void_p, PD, PD, PD, D]
+        L.orc_norm_phi_free.argtypes = [C.c_void_p, PD, PD, D]
+        L.orc_norm_phi_restr.argtypes = [C.c_void_p, PD, PD, PD, D]
         _lib = L
     return _lib
 
@@ -124,6 +131,40 @@ class Lattice:
         out = np.zeros_like(Gin)
         lib().orc_propagate2(self.h, _p(Gin), _p(_f64(g)), _p(out), S, int(backward))
         return out
+
+    def lngn2(self, G):
+        return lib().orc_lngn2(self.h, _p(_f64(G)))
+
+    def propagate_force(self, Gin, g, f):
+        Gin = _f64(Gin).copy()
+        out = np.zeros(self.M)
+        lib().orc_propagate_force(self.h, _p(Gin), _p(_f64(g)), _p(out), float(f))
+        return out
+
+    def connect2(self, A, B, out, OUT=None):
+        """(out, OUT) after ConnectG with PHI; OUT None: no per-axis parts."""
+        out = _f64(out).copy()
+        OUT = None if OUT is None else _f64(OUT).copy()
+        lib().orc_connect2(self.h, _p(_f64(A)), _p(_f64(B)), _p(out), None if OUT is None else _p(OUT))
+        return out, OUT
+
+    def correct2(self, v, V, G):
+        v = _f64(v).copy()
+        V = None if V is None else _f64(V).copy()
+        lib().orc_correct2(self.h, _p(v), None if V is None else _p(V), _p(_f64(G)))
+        return v, V
+
+    def norm_phi_free(self, phi, PHI, C):
+        phi = _f64(phi).copy()
+        PHI = None if PHI is None else _f64(PHI).copy()
+        lib().orc_norm_phi_free(self.h, _p(phi), None if PHI is None else _p(PHI), float(C))
+        return phi, PHI
+
+    def norm_phi_restr(self, phi, PHI, Gi, C):
+        phi = _f64(phi).copy()
+        PHI = None if PHI is None else _f64(PHI).copy()
+        lib().orc_norm_phi_restr(self.h, _p(phi), None if PHI is None else _p(PHI), _p(_f64(Gi)), float(C))
+        return phi, PHI
 
     def ncomp2(self):
         return lib().orc_ncomp2(self.h)

@@ -364,3 +364,25 @@ double orc_lngn2(const orc_lat* L, const double* G) {
   free(c);
   return log(value / 6.0);
 }
+
+/* ConnectG with PHI (Lat3D2ndO.cpp:905-931): out += A^k B^k for every k, and
+ * OUT + kk*M, kk the axis of k (0 for k = 0, 5; 1 for 1, 4; 2 for 2, 3). */
+void orc_connect2(const orc_lat* L, const double* A, const double* B, double* out, double* OUT) {
+  long M = L->M;
+  for (int k = 0; k < 6; k++) {
+    int kk = (k == 0 || k == 5) ? 0 : (k == 1 || k == 4) ? 1 : 2;
+    for (long i = 0; i < M; i++) {
+      out[i] += A[k * M + i] * B[k * M + i];
+      if (OUT) OUT[kk * M + i] += A[k * M + i] * B[k * M + i];
+    }
+  }
+}
+
+/* CorrectDoubleCountG with PHI (Lat3D2ndO.cpp:988-994, tools.cpp:86-90) */
+void orc_correct2(const orc_lat* L, double* in, double* IN, const double* G) {
+  long M = L->M;
+  for (long i = 0; i < M; i++) in[i] = G[i] > 0 ? in[i] / G[i] : 0.0;
+  if (IN)
+    for (int k = 0; k < 3; k++)
+      for (long i = 0; i < M; i++) IN[k * M + i] = G[i] > 0 ? IN[k * M + i] / G[i] : 0.0;
+}

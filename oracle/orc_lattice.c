@@ -17,6 +17,9 @@
  *   ComputeLnGN         Lat1DFlat1stO.cpp:301-316, Lat1DCyl1stO.cpp:258-270,
  *                       Lat3D1stO.cpp:345-370
  *   SubtractBoundaries  Lat1DFlat.cpp:409-415, Lat3D.cpp:1591-1640
+ *   force PropagateG    Lat1DFlat1stO.cpp:110-130, Lat3D1stO.cpp:94-165,
+ *                       Lat3DFCC1stO.cpp:102-145, Lat3DHEX1stO.cpp:102-145
+ *   NormPhiFree/Restr   Lat1DFlat1stO.cpp:318-331, Lat3D2ndO.cpp:1027-1047
  *   SideFraction/Sides  Lat1DFlat.cpp:510-524, Lat1DCylinder.cpp:160-171,
  *                       Lat3D.cpp:1729-1784
  *   2-D lattices        Lat2DFlat.cpp, Lat2DFlat1stO.cpp:31-52,
@@ -521,4 +524,94 @@ void orc_side(const orc_lat* L, double* P, double* out) {
         out[r] = s;
       }
   orc_zero_halos(L, out);
+}
+
+/* ---- force ensemble: PropagateG(Gi, G, Gout, f) --------------------------
+ * Lat1DFlat1stO.cpp:110-130: norm = l1 (e^-f + e^f - 2) + 1,
+ *   f1 = l1 e^f / norm on G(z-1), f_1 = l1 e^-f / norm on G(z+1),
+ *   Gout(z) = G(z) (b + f1 (a - b) + f_1 (c - b)).
+ * Lat3D1stO.cpp:94-165 (simple cubic): fnorm = 1 / (4 + e^f + e^-f); the
+ *   neighbour at z-1 weighs e^f, at z+1 e^-f, the four others 1.
+ * Lat3DFCC1stO.cpp:102-145, Lat3DHEX1stO.cpp:102-145: the same weights by the
+ *   z offset of each of the twelve neighbours, fnorm = 1 / (4 + 4 e^f + 4 e^-f)
+ *   for both lattices. */
+void orc_propagate_force(const orc_lat* L, double* Gin, const double* g, double* Gout, double f) {
+  orc_setb(L, Gin);
+  const double ef = exp(f), emf = exp(-f);
+  if (L->grad == 1) {
+    const double norm = L->l1 * (exp(-f) + exp(f) - 2) + 1;
+    const double f_1 = L->l1 * exp(-f) / norm, f1 = L->l1 * exp(f) / norm;
+    int M = L->MX;
+    double a, b = Gin[0], c = Gin[1];
+    for (int i = 1; i < M - 1; i++) {
+      a = b;
+      b = c;
+      c = Gin[i + 1];
+      Gout[i] = g[i] * (b + f1 * (a - b) + f_1 * (c - b));
+    }
+    orc_setb(L, Gout);
+    return;
+  }
+  long jx = L->jx, jy = L->jy, jz = 1;
+  const double fnorm = L->type == SFX_LAT_STANDARD ? 1.0 / (4.0 + ef + emf) : 1.0 / (4.0 + 4.0 * ef + 4.0 * emf);
+  for (int x = 1; x <= L->n[0]; x++)
+    for (int y = 1; y <= L->n[1]; y++)
+      for (int z = 1; z <= L->n[2]; z++) {
+        long r = (x * (long)L->MY + y) * L->MZ + z;
+        double s;
+        if (L->type == SFX_LAT_FCC) {
+          s = g[r] * Gin[r - jx - jy];
+          s += g[r] * Gin[r + jx + jy];
+          s += g[r] * Gin[r + jy - jx];
+          s += g[r] * Gin[r + jx - jy];
+          s += ef * g[r] * Gin[r - jy - jz];
+          s += emf * g[r] * Gin[r + jy + jz];
+          s += emf * g[r] * Gin[r + jz - jy];
+          s += ef * g[r] * Gin[r + jy - jz];
+          s += ef * g[r] * Gin[r - jx - jz];
+          s += emf * g[r] * Gin[r + jx + jz];
+          s += emf * g[r] * Gin[r + jz - jx];
+          s += ef * g[r] * Gin[r + jx - jz];
+        } else if (L->type == SFX_LAT_HEX) {
+          s = g[r] * Gin[r - jx];
+          s += g[r] * Gin[r + jx];
+          s += g[r] * Gin[r + jy - jx];
+          s += g[r] * Gin[r + jy];
+          s += g[r] * Gin[r - jy];
+          s += g[r] * Gin[r + jx - jy];
+          s += ef * g[r] * Gin[r + jy - jz];
+          s += ef * g[r] * Gin[r - jz];
+          s += ef * g[r] * Gin[r + jx - jz];
+          s += emf * g[r] * Gin[r + jz - jx];
+          s += emf * g[r] * Gin[r + jz];
+          s += emf * g[r] * Gin[r + jz - jy];
+        } else {
+          s = Gin[r - jx] * g[r];
+          s += Gin[r + jx] * g[r];
+          s += Gin[r - jy] * g[r];
+          s += Gin[r + jy] * g[r];
+          s += ef * Gin[r - jz] * g[r];
+          s += emf * Gin[r + jz] * g[r];
+        }
+        Gout[r] = s * fnorm;
+      }
+  orc_setb(L, Gout);
+}
+
+/* ---- normalisations (Lat1DFlat1stO.cpp:318-331, Lat3D2ndO.cpp:1027-1047) -
+ * NormPhiFree: phi *= C on every site (and the three axis parts of PHI).
+ * NormPhiRestr: C /= exp(ComputeLnGN(Gi)) first; on a 3-D second-order
+ * lattice C /= 6 exp(ComputeLnGN(Gi)) with the six-component ComputeLnGN. */
+void orc_norm_phi_free(const orc_lat* L, double* phi, double* PHI, double C) {
+  for (long i = 0; i < L->M; i++) phi[i] *= C;
+  if (PHI)
+    for (long i = 0; i < 3 * L->M; i++) PHI[i] *= C;
+}
+
+void orc_norm_phi_restr(const orc_lat* L, double* phi, double* PHI, const double* Gi, double C) {
+  if (L->order == 2)
+    C /= 6.0 * exp(orc_lngn2(L, Gi));
+  else
+    C /= exp(orc_lngn(L, Gi));
+  orc_norm_phi_free(L, phi, PHI, C);
 }

@@ -527,3 +527,108 @@ def test_cylinder_2d_order2_reduces_to_1d(orc, backward):
     for k5, k3 in enumerate([0, 1, 1, 1, 2]):
         np.testing.assert_allclose(out2[k5, 1:-1, 1:-1], np.repeat(out1[k3, 1:-1, None], ny, axis=1),
                                    rtol=1e-13, atol=0)
+
+
+# ---- force ensemble, normalisations, second-order connect (per-rank primitives)
+
+def _walk_moments(P, axis_coord):
+    m = P.sum()
+    mean = (P * axis_coord).sum() / m
+    var = (P * (axis_coord - mean) ** 2).sum() / m
+    return m, mean, var
+
+
+@pytest.mark.parametrize("f", [0.0, 0.3, -0.7])
+def test_force_1d_drift_closed_form(orc, f):
+    """A walk under a force on a flat 1-D lattice, g = 1, from one layer, far
+    from both walls: per step the mass is kept, the mean moves by f1 - f_1 and
+    the variance grows by f1 + f_1 - (f1 - f_1)^2, with f1 = l1 e^f / n,
+    f_1 = l1 e^-f / n, n = l1 (e^f + e^-f - 2) + 1 (Lat1DFlat1stO.cpp:110-130)."""
+    n, l1, steps, z0 = 200, 0.25, 40, 100
+    lat = LatticeSpec(1, (n,), lowerbound=("mirror1",), upperbound=("mirror1",), lambda_=l1)
+    L = orc.Lattice(lat)
+    G = np.zeros(lat.M)
+    G[z0] = 1.0
+    g = np.ones(lat.M)
+    for _ in range(steps):
+        G = L.propagate_force(G, g, f)
+    norm = l1 * (math.exp(f) + math.exp(-f) - 2) + 1
+    f1, f_1 = l1 * math.exp(f) / norm, l1 * math.exp(-f) / norm
+    z = np.arange(lat.M, dtype=float)
+    inner = slice(1, n + 1)
+    m, mean, var = _walk_moments(G[inner], z[inner])
+    assert abs(m - 1) < 1e-12
+    assert abs(mean - (z0 + steps * (f1 - f_1))) < 1e-10
+    assert abs(var - steps * (f1 + f_1 - (f1 - f_1) ** 2)) < 1e-9
+    if f == 0.0:  # no force: the plain propagator
+        G0 = np.zeros(lat.M)
+        G0[z0] = 1.0
+        np.testing.assert_allclose(L.propagate_force(G0, g, 0.0), L.propagate(G0, g), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("kind, n_dn, n_up, n_in, norm", [
+    ("standard", 1, 1, 4, lambda ef: 4 + ef + 1 / ef),
+    ("fcc", 4, 4, 4, lambda ef: 4 + 4 * ef + 4 / ef),
+    ("hex", 3, 3, 6, lambda ef: 4 + 4 * ef + 4 / ef)])
+def test_force_3d_drift_closed_form(orc, kind, n_dn, n_up, n_in, norm):
+    """A walk under a force along z on a periodic 3-D lattice, g = 1, from one
+    site: each step multiplies the mass by (n_in + n_dn e^f + n_up e^-f) fnorm
+    and moves the mean z by (n_dn e^f - n_up e^-f) / (n_in + n_dn e^f + n_up
+    e^-f), n_dn / n_up the neighbours one layer below / above, fnorm the
+    reference's normaliser. Simple cubic and FCC keep the mass; HEX, whose
+    normaliser the reference takes from FCC, does not."""
+    f = 0.4
+    ef = math.exp(f)
+    n, steps = (12, 12, 40), 8
+    lat = LatticeSpec(3, n, lattice_type=kind, lowerbound=("periodic",) * 3, upperbound=("periodic",) * 3)
+    L = orc.Lattice(lat)
+    G = np.zeros(lat.M)
+    MX, MY, MZ = lat.dims
+    r0 = (6 * MY + 6) * MZ + 20
+    G[r0] = 1.0
+    g = np.ones(lat.M)
+    for _ in range(steps):
+        G = L.propagate_force(G, g, f)
+    w = n_in + n_dn * ef + n_up / ef
+    zc = np.tile(np.arange(MZ, dtype=float), MX * MY)
+    inner = lat.interior_mask().reshape(-1)
+    m, mean, _ = _walk_moments(G[inner], zc[inner])
+    assert abs(m - (w / norm(ef)) ** steps) < 1e-12
+    assert abs(mean - (20 + steps * (n_dn * ef - n_up / ef) / w)) < 1e-10
+
+
+def test_norm_phi_restricted_gives_theta(orc):
+    """NormPhiRestr of phi = Gi puts exactly C into sum_z w(z) phi(z) (the
+    ComputeLnGN weights); second order with phi = sum_k Gi^k likewise, and the
+    per-axis parts scale with it. NormPhiFree is a plain scale."""
+    rng = np.random.default_rng(3)
+    lat = LatticeSpec(1, (30,), geometry="cylindrical", lowerbound=("mirror1",), upperbound=("mirror2",),
+                      lambda_=1 / 6)
+    L = orc.Lattice(lat)
+    Gi = rng.random(lat.M) + 0.1
+    phi, _ = L.norm_phi_restr(Gi, None, Gi, 2.5)
+    assert abs(math.exp(L.lngn(phi)) - 2.5) < 1e-12
+    lat2 = LatticeSpec(3, (5, 6, 4), order=2, lowerbound=("mirror1", "periodic", "mirror2"),
+                       upperbound=("mirror2", "periodic", "mirror1"))
+    L2 = orc.Lattice(lat2)
+    Gi6 = rng.random(6 * lat2.M) + 0.1
+    tot = Gi6.reshape(6, -1).sum(axis=0)
+    PHI = rng.random(3 * lat2.M)
+    phi2, PHI2 = L2.norm_phi_restr(tot, PHI, Gi6, 7.0)
+    # six equal components phi2 / 6: the six-component ComputeLnGN sums w phi2
+    assert abs(6 * math.exp(L2.lngn2(np.tile(phi2, 6) / 6)) - 7.0) < 1e-11
+    np.testing.assert_allclose(PHI2 / PHI, phi2[0] / tot[0], rtol=1e-14)
+    p3, P3 = L.norm_phi_free(Gi, None, 0.5)
+    np.testing.assert_array_equal(p3, Gi * 0.5)
+
+
+def test_connect2_axis_parts_add_up(orc):
+    """ConnectG with PHI: the three per-axis parts add up to the total added."""
+    rng = np.random.default_rng(4)
+    lat = LatticeSpec(3, (4, 5, 3), order=2, lowerbound=("periodic",) * 3, upperbound=("periodic",) * 3)
+    L = orc.Lattice(lat)
+    A, B = rng.random(6 * lat.M), rng.random(6 * lat.M)
+    out0 = rng.random(lat.M)
+    out, OUT = L.connect2(A, B, out0, np.zeros(3 * lat.M))
+    np.testing.assert_allclose(OUT.reshape(3, -1).sum(axis=0), out - out0, rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(out - out0, (A * B).reshape(6, -1).sum(axis=0), rtol=1e-14)
