@@ -39,7 +39,22 @@ struct Tangent {
   long long M = 0;
   int maxstore = 0;
   DevBuf<double> dstore, dpp, dacc, dphi_m, dphi, drho, dphitot, dmol, v64, v32, partial2;
-  DevBuf<double> adj_a, adj_b;  // VJP work arrays
+  // transposed products (allocated by the first VJP): per-state adjoints of
+  // the contact averages, of phi and of the per-state direction, the total's
+  // adjoint, and the per-state sweep direction (FP64 and FP32)
+  DevBuf<double> rbar, phibar, vbar, ptbar, xdir64, xdir32, vpart;
+  bool vjp_ready = false;
+  void init_vjp(System& S) {
+    const size_t nsM = (size_t)S.ns * M;
+    rbar.alloc(nsM);
+    phibar.alloc(nsM);
+    vbar.alloc(nsM);
+    ptbar.alloc((size_t)M);
+    xdir64.alloc(nsM);
+    xdir32.alloc(nsM);  // half used
+    vpart.alloc((size_t)SFX_MAXM * S.npart);
+    vjp_ready = true;
+  }
   DevBuf<double> dacc2, dpp2;   // the backward chain's, in a middle-out sweep
   void init(System& S) {
     M = S.lat.d.M;
@@ -973,12 +988,317 @@ void jvp_internal(System& S, const double* v, double* jv, int precision) {
   jvp_at_base(S, v, jv, precision);
 }
 
+// ---------------------------------------------------------------------------
+// Transposed products J^T w. The composition of a chain is the derivative of
+// its single-chain partition function, phi_m,t(r) = -(pref / D(r)) dQ/du_t(r),
+// wherever the backward sweep is the adjoint of the forward sweep in the
+// site-volume inner product D (D = the ComputeLnGN weight of a site: 1, 1/2
+// on a mirror2 boundary layer, L(z) on curved lattices). The tangent sweep's
+// map T: v -> dacc / g then satisfies T^T(y) = D T(y / D) (the Hessian of Q
+// is symmetric), so the transposed sweep is the tangent sweep run with the
+// direction y / D. The contact average is symmetric in the same inner product
+// (Lat1DFlat.cpp:510-524, Lat1DCylinder.cpp:160-171, Lat3D.cpp:1729-1784), so
+// its transpose is D <. / D>. Everything else of the residual
+// (SF_Solve.cpp:554-590), the normalisations and the solvent closure is
+// transposed term by term, in the reverse order of the forward product.
+
+// true where the backward sweep is the forward sweep's adjoint in D. Not so:
+// the 2-D cylinder's second-order sweep (its outward step is not split, as in
+// the reference); a curved lattice with a mirror2 side; a 1-D flat lattice
+// with a layer adjustment other than 1; a mirror on a HEX lattice (its
+// neighbour set is not symmetric under reflection, so the mirrored stencil is
+// not); a second-order 1-D or 2-D lattice with a mirror2 lower bound (the
+// reference copies the components there without reversing them).
+bool vjp_supported(const System& S) {
+  const DevLattice& L = S.lat.d;
+  if (L.curved) {
+    for (int k = 0; k < L.grad; k++)
+      if (L.mirror2_1[k] || L.mirror2_2[k]) return false;
+    if (L.grad == 2 && L.order == 2) return false;
+  }
+  if (L.grad == 1 && !L.curved && L.ladj != 1.0) return false;
+  if (L.kind == ST_3D_HEX)
+    for (int k = 0; k < 3; k++)
+      if (!L.periodic[k]) return false;
+  if (L.order == 2 && L.grad < 3)
+    for (int k = 0; k < L.grad; k++)
+      if (L.mirror2_1[k]) return false;
+  return true;
+}
+
+// per site: w-hat = w - mean_k w (the transposed mean subtraction); the direct
+// part of J^T w (w-hat); the adjoint of every state's contact average; the
+// adjoint of phi_total; and per molecule the workgroup's part of the adjoint of
+// its bulk fraction through ChemIntBulk
+__global__ void k_vjp_res(DevLattice L, SysParams sp, const unsigned char* __restrict__ itmask,
+                          const double* __restrict__ w, const double* __restrict__ rho,
+                          const double* __restrict__ phitot, double* __restrict__ jtw, double* __restrict__ rbar,
+                          double* __restrict__ ptbar, double* __restrict__ part) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nv = sp.n_itvar;
+  double cb[SFX_MAXM];
+  for (int m = 0; m < SFX_MAXM; m++) cb[m] = 0.0;
+  if (i < L.M) {
+    if (!itmask[i]) {
+      for (int k = 0; k < nv; k++) jtw[(long long)k * L.M + i] = 0.0;
+      for (int j = 0; j < sp.ns; j++) rbar[(long long)j * L.M + i] = 0.0;
+      ptbar[i] = 0.0;
+    } else {
+      const double pt = phitot[i];
+      double wsum = 0.0;
+      for (int k = 0; k < nv; k++) wsum += w[(long long)k * L.M + i];
+      const double mean = wsum / nv;
+      double pbar = -wsum / (pt * pt);
+      for (int j = 0; j < sp.ns; j++) rbar[(long long)j * L.M + i] = 0.0;
+      for (int k = 0; k < nv; k++) {
+        const double wh = w[(long long)k * L.M + i] - mean;
+        jtw[(long long)k * L.M + i] = wh;
+        const int t = sp.itvar_rep[k];
+        double ci = 0.0;
+        for (int j = 0; j < sp.ns; j++) {
+          const double c = sp.chi[t * SFX_MAXS + j];
+          if (c == 0) continue;
+          ci += c * rho[(long long)j * L.M + i];
+          rbar[(long long)j * L.M + i] -= wh * c / pt;
+          if (!sp.melt && !sp.frozen[j])
+            for (int m = 0; m < sp.nm; m++) cb[m] += wh * c * sp.mol_frac[m * SFX_MAXS + j];
+        }
+        pbar += wh * ci / (pt * pt);
+      }
+      ptbar[i] = pbar;
+    }
+  }
+  __shared__ double red[SFX_MAXM][kTB / 64];
+  for (int m = 0; m < sp.nm; m++) {
+    double v = cb[m];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[m][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < (unsigned)sp.nm) {
+    double s = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += red[threadIdx.x][k];
+    part[(long long)threadIdx.x * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// q_t = rbar_t / D on the interior, halos from their sources (0 behind a
+// surface), frozen states 0: the input of the transposed contact average
+__global__ void k_vjp_q(DevLattice L, int ns, const int* __restrict__ frozen, const double* __restrict__ rbar,
+                        double* __restrict__ q) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.M) return;
+  int x, y, z;
+  site_coords(L, i, x, y, z);
+  const int sx = remap(x, L.nx, L.b1[0], L.b2[0]);
+  const int sy = L.grad >= 2 ? remap(y, L.ny, L.b1[1], L.b2[1]) : y;
+  const int sz = L.grad >= 3 ? remap(z, L.nz, L.b1[2], L.b2[2]) : z;
+  const bool surf = !is_interior(L, sx, sy, sz);
+  const long long s = sidx(L, sx, sy, sz);
+  const double d = surf ? 1.0 : lngn_weight(L, sx, sy, sz);
+  for (int t = 0; t < ns; t++)
+    q[(long long)t * L.M + i] = (frozen[t] || surf) ? 0.0 : rbar[(long long)t * L.M + s] / d;
+}
+
+// phibar_t = D <q_t> + ptbar on the interior (non-frozen states), 0 elsewhere
+__global__ void k_vjp_phibar(DevLattice L, int ns, const int* __restrict__ frozen, const double* __restrict__ q,
+                             const double* __restrict__ ptbar, double* __restrict__ phibar) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.M) return;
+  int x, y, z;
+  site_coords(L, i, x, y, z);
+  const bool in = is_interior(L, x, y, z);
+  const double d = in ? lngn_weight(L, x, y, z) : 0.0;
+  for (int t = 0; t < ns; t++)
+    phibar[(long long)t * L.M + i] =
+        (in && !frozen[t]) ? d * side_at(L, q + (long long)t * L.M, x, y, z) + ptbar[i] : 0.0;
+}
+
+// per-block sums over the interior of sum_{t in uses} a_t b_t (with D when weighted)
+__global__ void k_vjp_dot(DevLattice L, unsigned uses, int ns, const double* __restrict__ a,
+                          const double* __restrict__ b, int weighted, double* __restrict__ part) {
+  const long long q = site_ordinal();
+  double v = 0.0;
+  if (q < L.V) {
+    int x, y, z;
+    interior_coords(L, q, x, y, z);
+    const long long r = sidx(L, x, y, z);
+    for (int t = 0; t < ns; t++)
+      if ((uses >> t) & 1u) v += a[(long long)t * L.M + r] * (b ? b[(long long)t * L.M + r] : 1.0);
+    if (weighted) v *= lngn_weight(L, x, y, z);
+  }
+  __shared__ double red[kTB / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += red[k];
+    part[blockIdx.x] = s;
+  }
+}
+
+// vbar_t += a phi_m,t phibar_t + c D phi_m,t on the interior (states in uses)
+__global__ void k_vjp_direct(DevLattice L, unsigned uses, int ns, const double* __restrict__ phi_m,
+                             const double* __restrict__ phibar, double a, double c, double* __restrict__ vbar) {
+  const long long q = site_ordinal();
+  if (q >= L.V) return;
+  int x, y, z;
+  interior_coords(L, q, x, y, z);
+  const long long r0 = sidx(L, x, y, z);
+  const double d = lngn_weight(L, x, y, z);
+  for (int t = 0; t < ns; t++) {
+    if (!((uses >> t) & 1u)) continue;
+    const long long r = (long long)t * L.M + r0;
+    vbar[r] += phi_m[r] * (a * phibar[r] + c * d);
+  }
+}
+
+// the sweep direction x_t = phibar_t / D on the interior for t in uses, else 0
+__global__ void k_vjp_dir(DevLattice L, unsigned uses, int ns, const double* __restrict__ phibar,
+                          double* __restrict__ x64, float* __restrict__ x32) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.M) return;
+  int x, y, z;
+  site_coords(L, i, x, y, z);
+  const bool in = is_interior(L, x, y, z);
+  const double d = in ? lngn_weight(L, x, y, z) : 1.0;
+  for (int t = 0; t < ns; t++) {
+    const double v = (in && ((uses >> t) & 1u)) ? phibar[(long long)t * L.M + i] / d : 0.0;
+    x64[(long long)t * L.M + i] = v;
+    x32[(long long)t * L.M + i] = (float)v;
+  }
+}
+
+// vbar_t += D dm[3] (dacc_t + dacc2_t) / g_t: the transposed sweep's share
+template <typename T>
+__global__ void k_vjp_sweep_term(DevLattice L, unsigned uses, int ns, const T* __restrict__ dacc,
+                                 const T* __restrict__ dacc2, const double* __restrict__ g,
+                                 const double* __restrict__ dm, double* __restrict__ vbar) {
+  const long long q = site_ordinal();
+  if (q >= L.V) return;
+  int x, y, z;
+  interior_coords(L, q, x, y, z);
+  const long long r0 = sidx(L, x, y, z);
+  const double d = lngn_weight(L, x, y, z);
+  for (int t = 0; t < ns; t++) {
+    if (!((uses >> t) & 1u)) continue;
+    const long long r = (long long)t * L.M + r0;
+    const double gv = g[r];
+    if (gv > 0) vbar[r] += d * dm[3] * ((double)dacc[r] + (dacc2 ? (double)dacc2[r] : 0.0)) / gv;
+  }
+}
+
+// J^T w = w-hat + sum over the states of each iteration variable of vbar, on
+// the iteration sites (the transpose of k_state_dir)
+__global__ void k_vjp_final(long long M, int nv, int ns, const int* __restrict__ st_itvar,
+                            const unsigned char* __restrict__ itmask, const double* __restrict__ vbar,
+                            double* __restrict__ jtw) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  if (!itmask[i]) return;  // k_vjp_res zeroed these
+  for (int t = 0; t < ns; t++) {
+    const int k = st_itvar[t];
+    if (k >= 0) jtw[(long long)k * M + i] += vbar[(long long)t * M + i];
+  }
+}
+
+static double host_sum(const double* part, int n, hipStream_t st) {
+  std::vector<double> h(n);
+  SFX_HIP(hipMemcpyAsync(h.data(), part, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  SFX_HIP(hipStreamSynchronize(st));
+  double s = 0.0;
+  for (double v : h) s += v;
+  return s;
+}
+
+// J^T w at the base state of the last compute_phi at S.d_u
+void vjp_at_base(System& S, const double* w, double* jtw, int precision) {
+  SFX_REQUIRE(vjp_supported(S), SFX_E_UNSUPPORTED,
+              "transposed products need a backward sweep that is the forward sweep's adjoint: not on a curved "
+              "lattice with a mirror2 side, the second-order 2-D cylinder, a layer adjustment other than 1, a "
+              "mirror on a HEX lattice or a second-order 1-D/2-D mirror2 lower bound");
+  if (!S.tan) {
+    S.tan = new Tangent();
+    S.tan->init(S);
+  }
+  Tangent& W = *S.tan;
+  if (!W.vjp_ready) W.init_vjp(S);
+  const DevLattice& L = S.lat.d;
+  const long long Mn = L.M;
+  hipStream_t st = S.stream;
+  const bool fp32 = precision == SFX_FP32;
+  const unsigned nbM = grid1(Mn, kTB), nbV = grid1(L.V, kTB);
+  // residual: the direct part, the contact averages' and the total's adjoints,
+  // and each molecule's bulk-fraction adjoint through ChemIntBulk
+  k_vjp_res<<<nbM, kTB, 0, st>>>(L, S.sp, S.d_itmask, w, S.d_rho, S.d_phitot, jtw, W.rbar.p, W.ptbar.p, W.vpart.p);
+  std::vector<double> pbbar(S.nm, 0.0);
+  for (int m = 0; m < S.nm; m++) pbbar[m] = host_sum(W.vpart.p + (size_t)m * nbM, (int)nbM, st);
+  // the contact averages and phi_total, transposed
+  k_vjp_q<<<nbM, kTB, 0, st>>>(L, S.ns, S.d_frozen, W.rbar.p, W.xdir64.p);
+  k_vjp_phibar<<<nbM, kTB, 0, st>>>(L, S.ns, S.d_frozen, W.xdir64.p, W.ptbar.p, W.phibar.p);
+  SFX_HIP(hipMemsetAsync(W.vbar.p, 0, (size_t)S.ns * Mn * sizeof(double), st));
+  std::vector<double> molscal((size_t)S.nm * MS_COUNT);
+  SFX_HIP(hipMemcpyAsync(molscal.data(), S.d_molscal, molscal.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  SFX_HIP(hipStreamSynchronize(st));
+  if (fp32 && S.use_block3 && !S.gmax_valid) {
+    launch_gmax(L, S.d_swf32, S.ns, S.d_gmax, st);
+    S.gmax_valid = true;
+  }
+  // molecules in the reverse order of the forward product: the solvent first
+  // (its bulk fraction closes on the others'), then the rest
+  for (int pass = 0; pass < 2; pass++)
+    for (int m = 0; m < S.nm; m++) {
+      const Molecule& mo = S.mols[m];
+      const bool solv = mo.freedom == SFX_SOLVENT;
+      if ((pass == 0) != solv) continue;
+      const unsigned uses = mo.uses_mask();
+      const double* phi_m = S.d_phi_m + (size_t)m * S.ns * Mn;
+      k_vjp_dot<<<nbV, kTB, 0, st>>>(L, uses, S.ns, phi_m, W.phibar.p, 0, W.vpart.p);
+      const double relbar = host_sum(W.vpart.p, (int)nbV, st);
+      const double phib = molscal[(size_t)m * MS_COUNT + MS_PHIB];
+      const bool monomer = mo.kind == 0;
+      double lnbar = 0.0;
+      if (solv) {
+        // rel = dphib / phib, dphib = -sum_{m' != solvent} dphib_m'
+        const double dpb = (phib != 0 ? relbar / phib : 0.0) + pbbar[m];
+        for (int o = 0; o < S.nm; o++)
+          if (o != m) pbbar[o] -= dpb;
+      } else if (mo.freedom == SFX_RESTRICTED) {
+        // rel = -dlnGN, dphib = -phib dlnGN
+        lnbar = -relbar - phib * pbbar[m];
+      }
+      // dlnGN = -(N / theta) sum_t,r D phi_m,t v_t, theta = sum D phi_m
+      double coef = 0.0;
+      if (lnbar != 0.0) {
+        k_vjp_dot<<<nbV, kTB, 0, st>>>(L, uses, S.ns, phi_m, nullptr, 1, W.vpart.p);
+        const double theta = host_sum(W.vpart.p, (int)nbV, st);
+        coef = -lnbar * mo.N / theta;
+      }
+      k_vjp_direct<<<nbV, kTB, 0, st>>>(L, uses, S.ns, phi_m, W.phibar.p, monomer ? -1.0 : 1.0, coef, W.vbar.p);
+      if (monomer) continue;
+      // the transposed sweep: the tangent sweep with direction phibar / D
+      k_vjp_dir<<<nbM, kTB, 0, st>>>(L, uses, S.ns, W.phibar.p, W.xdir64.p, (float*)W.xdir32.p);
+      const size_t off = (size_t)m * S.ns * Mn;
+      if (fp32) {
+        tsweep<float>(S, W, m, true, (const float*)W.xdir32.p, S.d_swf32, W.xdir64.p);
+        const bool two = S.middle_out(mo);
+        k_vjp_sweep_term<float><<<nbV, kTB, 0, st>>>(L, uses, S.ns, (const float*)W.dacc.p + off,
+                                                     two ? (const float*)W.dacc2.p + off : nullptr, S.d_swf,
+                                                     W.dmol.p + m * 4, W.vbar.p);
+      } else {
+        tsweep<double>(S, W, m, false, W.xdir64.p, S.d_swf, W.xdir64.p);
+        k_vjp_sweep_term<double><<<nbV, kTB, 0, st>>>(L, uses, S.ns, (const double*)W.dacc.p + off, nullptr,
+                                                      S.d_swf, W.dmol.p + m * 4, W.vbar.p);
+      }
+    }
+  k_vjp_final<<<nbM, kTB, 0, st>>>(Mn, S.n_itvar, S.ns, S.d_st_itvar, S.d_itmask, W.vbar.p, jtw);
+  SFX_HIP(hipGetLastError());
+}
+
 void vjp_internal(System& S, const double* w, double* jtw, int precision) {
-  (void)S;
-  (void)w;
-  (void)jtw;
-  (void)precision;
-  throw Error{SFX_E_UNSUPPORTED, "transpose products are not built yet"};
+  S.compute_phi(precision);
+  vjp_at_base(S, w, jtw, precision);
 }
 
 // ABI-layout vector (host or device) -> internal [itvar][M] (zero off sites)

@@ -4,10 +4,10 @@
 device tensors and is a torch.autograd.Function whose forward-mode derivative
 is the hand-written tangent propagator sweep (`sfx_jvp`): under
 `torch.autograd.forward_ad` (or `torch.func.jvp`) the tangent of f is the exact
-Jacobian-vector product J(u) v, not a finite difference. The iteration vector
-uses the layout of `sfx_residual` (the reference's SF_Solve / MPS layout).
-Reverse mode needs J^T w, which the library does not provide; `backward`
-raises.
+Jacobian-vector product J(u) v, not a finite difference. Its reverse-mode
+derivative is the transposed product J(u)^T w (`sfx_vjp`), so `backward()` and
+`torch.autograd.grad` work as well. The iteration vector uses the layout of
+`sfx_residual` (the reference's SF_Solve / MPS layout).
 """
 import torch
 
@@ -33,7 +33,7 @@ class _Residual(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad):
-        raise NotImplementedError("reverse mode (J^T w) is not provided; use forward-mode AD (J v)")
+        return ctx.system.vjp(ctx.u, grad.detach().contiguous(), precision=ctx.precision), None, None
 
 
 def residual(u: torch.Tensor, system: "sfx.System", precision: str = "fp64") -> torch.Tensor:

@@ -163,7 +163,16 @@ int sfx_residual(sfx_system* s, const double* x, double* f, int where, int preci
 /* Jacobian-vector product J(x) v of the residual, by tangent propagators. */
 int sfx_jvp(sfx_system* s, const double* x, const double* v, double* jv, int where,
             int precision);
-/* Transposed product J(x)^T w, by adjoint propagators. */
+/* Transposed product J(x)^T w. The composition of a chain is the derivative
+ * of its partition function wherever the backward sweep is the forward sweep's
+ * adjoint in the site-volume inner product, so the transposed sweep is the
+ * tangent sweep with a rescaled direction. SFX_E_UNSUPPORTED where that does
+ * not hold: a curved lattice with a mirror2 side, the second-order 2-D
+ * cylinder (the reference's backward sweep does not split its outward step,
+ * Lat2DCyl2ndO.cpp:223-227), a 1-D flat layer adjustment other than 1, a
+ * mirror side on a HEX lattice (its neighbour set is not reflection
+ * symmetric), a second-order 1-D or 2-D lattice with a mirror2 lower bound
+ * (copied without reversing the components, as the reference does). */
 int sfx_vjp(sfx_system* s, const double* x, const double* w, double* jtw, int where,
             int precision);
 /* Results of the last sfx_compute_phi / sfx_residual. */

@@ -53,9 +53,10 @@ def test_forward_ad_fp32_and_errors(sfx):
     err = (j32 - j64).abs().max().item() / j64.abs().max().item()
     print(f"FP32 tangent sweep vs FP64: rel err {err:.2e}")
     assert err < 1e-3
-    f = torch_ops.residual(u.clone().requires_grad_(True), G)
-    with pytest.raises(NotImplementedError):
-        f.sum().backward()
+    uu = u.clone().requires_grad_(True)
+    torch_ops.residual(uu, G).sum().backward()  # reverse mode: J^T 1 (sfx_vjp)
+    ref = torch.from_numpy(G.vjp(x, np.ones(G.nvar))).cuda()
+    assert torch.allclose(uu.grad, ref, rtol=1e-12, atol=1e-14)
     with pytest.raises(TypeError):
         torch_ops.residual(u.cpu(), G)
 
