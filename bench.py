@@ -24,6 +24,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -106,6 +108,65 @@ def cpu_baseline(args, evals_needed):
                       f"{evals_needed} residual evaluations of the plain DIIS chi sweep"}
 
 
+def batch_leg(args):
+    """The 512 uneven 1-D adsorption problems (layers 50..400, N 20..1000) in
+    one persistent launch, DIIS with per-problem backtracking to |f| < 1e-7:
+    time per batched solve (HIP events around the launch), convergence, and the
+    algorithmic bytes the launch moves against the HBM roofline. Per residual a
+    problem writes and reads back its N/2 stored ranks (16 B per site-rank) and
+    reads x and writes f (32 B per layer); the rank loop itself runs in LDS."""
+    from field_amd import configs, sfx
+    probs = configs.batch_1d(512)
+    B = sfx.Batch(probs)
+    B.solve(tolerance=1e-7, max_iter=args.batch_max_iter, diis_backtrack=1, diis_eta=1.0)  # warm-up
+    x, its, norms, st = B.solve(tolerance=1e-7, max_iter=args.batch_max_iter, diis_backtrack=1, diis_eta=1.0)
+    evals = its.astype(float) + 1
+    per_eval = [(p["N"] // 2) * p["n_layers"] * 16.0 + 32.0 * p["n_layers"] for p in probs]
+    nbytes = float(np.dot(evals, per_eval))
+    conv = norms <= 1e-7
+    stable = np.array([configs.bulk_is_stable(p) for p in probs])
+    threads, tuned = B.tuning()
+    return {"workload": "512 x Lat1DFlat1stO adsorption, layers U{50..400}, N U{20..1000}, chi_AS U[0,1], "
+                        "chi_s U[-2,0], phi_b 10^U[-4,-1] (seeds 4..515), FP64, DIIS m=10 eta=1 with backtracking",
+            "seconds_per_batched_solve": round(st["seconds"], 4), "converged": int(conv.sum()),
+            "stable_bulk": int(stable.sum()), "stable_bulk_converged": int(conv[stable].sum()),
+            "max_iterations": int(its.max()), "residual_evaluations": int(evals.sum()),
+            "threads_per_problem": threads, "tuned_residual_ms": [round(t, 3) for t in tuned],
+            "algorithmic_bytes": nbytes,
+            "achieved_GBs": round(nbytes / st["seconds"] / 1e9, 1),
+            "frac_of_hbm_peak": round(nbytes / st["seconds"] / 1e9 / PEAK_HBM_GBS, 4),
+            "bound": "latency: a chain of N barriered rank steps per residual in each workgroup"}
+
+
+def transfer_compare(args):
+    """The 128^3 semiflexible system (A_200, restricted phi = 0.3, chi = 0.6,
+    stiffness 1 kT): FP32 residual + JVP with the second-order direction
+    transfer as structured rows and on the matrix cores; per-kernel average
+    launch times (HIP events) and wall time per residual + JVP."""
+    import numpy as np
+    from field_amd import configs, sfx
+    lat, sysd, x = configs.semiflexible_3d()
+    out = {}
+    for tk in ("rows", "mfma", "rows", "mfma"):
+        G = sfx.System(lat, sysd, transfer_kernel=tk)
+        v = np.random.default_rng(1).normal(0, 1, G.nvar)
+        G.residual(x, precision="fp32")
+        G.jvp(x, v, precision="fp32")
+        G.profile_start()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            G.residual(x, precision="fp32")
+            G.jvp(x, v, precision="fp32")
+        t1 = time.perf_counter()
+        prof = G.profile_stop()
+        rec = {f"{p['kernel']} {p['precision']} avg_us": round(p["avg_us"], 2) for p in prof}
+        rec["ms_per_residual_plus_jvp"] = round((t1 - t0) * 1e3 / args.steps, 2)
+        out.setdefault(tk, []).append(rec)
+        G.close()
+    print(json.dumps({"workload": "3-D SC 128^3 second order, A_200 restricted phi=0.3, chi=0.6, stiffness 1 kT; "
+                                  "FP32 residual + FP32 JVP", "transfer": out}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +188,10 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--chi-path", default="0.38,0.4")
     ap.add_argument("--direct", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 512-problem leg")
+    ap.add_argument("--batch-max-iter", type=int, default=5000)
+    ap.add_argument("--transfer-compare", action="store_true",
+                    help="only time the second-order transfer as rows and on the matrix cores (128^3), then exit")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one untimed FP32 elementwise copy of this many MiB first (a known-traffic "
                          "kernel that calibrates hardware counters read in the same run)")
@@ -135,6 +200,10 @@ def main():
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the solve is one lattice "
               f"on one device; multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.transfer_compare:
+        transfer_compare(args)
+        return
 
     import numpy as np
     from field_amd import configs, sfx
@@ -240,6 +309,8 @@ def main():
                     "launches": d["launches"], "ranks_per_launch": round(d["ranks"] / d["launches"], 3),
                     "bytes_per_launch": d["bytes_per_launch"]}
 
+    batch = None if args.no_batch else batch_leg(args)
+
     cpu = None
     if not args.no_cpu and bstats:
         evals = bstats[0]["residual_evals"]
@@ -281,6 +352,7 @@ def main():
         "parity": parity,
         "cpu_baseline": cpu,
         "direct_at_chi_0.4": direct,
+        "batched_adsorption": batch,
     }
     print(json.dumps(out))
 

@@ -66,11 +66,12 @@ class SystemDesc(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("memory_budget", C.c_size_t), ("sweep_kernel", C.c_int),
-                ("chain_schedule", C.c_int)]
+                ("chain_schedule", C.c_int), ("transfer_kernel", C.c_int)]
 
 
 SWEEP_AUTO, SWEEP_PER_RANK, SWEEP_BLOCKED = 0, 1, 2
 SCHEDULE_AUTO, SCHEDULE_SEQUENTIAL = 0, 1
+TRANSFER_AUTO, TRANSFER_ROWS, TRANSFER_MFMA = 0, 1, 2
 
 
 class SolverOpts(C.Structure):

@@ -77,6 +77,7 @@ struct DevLattice {
   int periodic[3];
   int flip1[3], flip2[3];  // 2nd order: a halo read reverses the axis' components
   int order;
+  int transfer_mfma;  // 2nd order, FP32: the direction transfer on the matrix cores (0 no, 1 yes, 2 tangents)
   double l1, l1chi, ladj;
   double l0_2d, l1x, l1y;
   // per-x-layer tables (device), curved lattices only

@@ -71,6 +71,46 @@ __device__ __forceinline__ void transfer_rows(T S, T Lw, const T in[6], T y[6]) 
   for (int k = 0; k < 6; k++) y[k] = S * in[k] + Lw * (sum - in[k] - in[5 - k]);
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// y = T in on the matrix cores, FP32: v_mfma_f32_4x4x1_16b_f32 runs 16
+// independent 4x4x1 products, block b on lanes 4b..4b+3. Lane 4b+j of a block
+// is site j of the block: per component c it gives B[0][j] = in[c] of its own
+// site and A[i][0] = T[i][c] for the row i = j it stands for, and the product
+// D[i][j] comes back in lane 4b+j, register i. Two accumulators hold rows 0-3
+// and rows 4-5 (rows 6-7 of the second are zero), so after six steps every
+// lane holds the six outputs of its own site: no data moves between lanes.
+// MFMA reads all 64 lanes, so every lane of the wave must call it (lanes
+// without a site pass zeros).
+__device__ __forceinline__ void transfer_mfma(float S, float Lw, const float in[6], float y[6]) {
+  const int i = threadIdx.x & 3;
+  f32x4 lo = {0.f, 0.f, 0.f, 0.f}, hi = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    const float tlo = i == c ? S : (i == 5 - c ? 0.f : Lw);
+    const int ih = i + 4;
+    const float thi = ih > 5 ? 0.f : (ih == c ? S : (ih == 5 - c ? 0.f : Lw));
+    lo = __builtin_amdgcn_mfma_f32_4x4x1f32(tlo, in[c], lo, 0, 0, 0);
+    hi = __builtin_amdgcn_mfma_f32_4x4x1f32(thi, in[c], hi, 0, 0, 0);
+  }
+  y[0] = lo[0];
+  y[1] = lo[1];
+  y[2] = lo[2];
+  y[3] = lo[3];
+  y[4] = hi[0];
+  y[5] = hi[1];
+}
+
+// the transfer of one site, on the matrix cores for FP32 when MFMA is set;
+// wave-uniform control flow is required in the MFMA case
+template <typename T, bool MFMA>
+__device__ __forceinline__ void transfer(T S, T Lw, const T in[6], T y[6]) {
+  if constexpr (MFMA && sizeof(T) == 4)
+    transfer_mfma(S, Lw, in, y);
+  else
+    transfer_rows(S, Lw, in, y);
+}
+
 // the six neighbours r + e_d of a site (e = {+x, +y, +z, -z, -y, -x}) with the
 // component reversal of a mirror halo, or -1 across a surface (nb2)
 __device__ __forceinline__ void neighbours(const DevLattice& L, int x, int y, int z, long long nbr[6],
@@ -157,7 +197,7 @@ __device__ __forceinline__ void pair_rank(const DevLattice& L, const T* __restri
 
 // forward: G^k_s(r) = g(r) (T G_{s-1})^k(r - e_k) from the stored W_{s-1};
 // the rank is stored as W_s = T G_s (tangent: dW_s = T dG_s)
-template <typename T, bool SCALED, bool TANGENT, bool CURVED>
+template <typename T, bool SCALED, bool TANGENT, bool CURVED, bool MFMA>
 __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restrict__ Win, const T* __restrict__ dWin,
                                                 const T* __restrict__ g, const T* __restrict__ v,
                                                 T* __restrict__ Wout, T* __restrict__ dWout, T S, RankScale sc) {
@@ -200,23 +240,24 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
   }
   const T Lw = (T(1) - S) / T(4);
   float m = 0.f;
-  if (in) {
-    const T gv = graw * scale;
-    T G[6], dG[6];
+  // lanes without a site carry zeros through the transfer (it may run on the
+  // matrix cores, which need the whole wave)
+  const T gv = graw * scale;
+  T G[6], dG[6], w[6], dw[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-      G[k] = gv * a[k];
-      m = fmaxf(m, (float)G[k]);
-      if constexpr (TANGENT) dG[k] = gv * (da[k] - vv * a[k]);
-    }
-    T w[6];
-    transfer_rows(S, Lw, G, w);
+  for (int k = 0; k < 6; k++) {
+    G[k] = gv * a[k];
+    m = fmaxf(m, (float)G[k]);
+    dG[k] = TANGENT ? gv * (da[k] - vv * a[k]) : T(0);
+  }
+  transfer<T, MFMA>(S, Lw, G, w);
+  if constexpr (TANGENT) transfer<T, MFMA>(S, Lw, dG, dw);
+  if (in) {
 #pragma unroll
     for (int k = 0; k < 6; k++) Wout[(long long)k * M + r] = w[k];
     if constexpr (TANGENT) {
-      transfer_rows(S, Lw, dG, w);
 #pragma unroll
-      for (int k = 0; k < 6; k++) dWout[(long long)k * M + r] = w[k];
+      for (int k = 0; k < 6; k++) dWout[(long long)k * M + r] = dw[k];
     }
   }
   if constexpr (SCALED) {
@@ -235,7 +276,7 @@ __global__ void __launch_bounds__(kB2) k_prop2f(DevLattice L, const T* __restric
 // backward: out = g T z with z_j = G^j(r + e_j) (reversed component across a
 // mirror), then acc += c sum_k out^k G_f^k (tangent: dacc) with the forward
 // partner recovered from W_f(s-1)
-template <typename T, bool SCALED, bool TANGENT, bool CURVED>
+template <typename T, bool SCALED, bool TANGENT, bool CURVED, bool MFMA>
 __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restrict__ Gin, const T* __restrict__ dGin,
                                                 const T* __restrict__ g, const T* __restrict__ v,
                                                 T* __restrict__ Gout, T* __restrict__ dGout,
@@ -249,10 +290,10 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
   // they are in flight
   const T Lw = (T(1) - S) / T(4);
   long long r = 0;
-  T o[6], d[6], pa[6], dpa[6];
+  T o[6], d[6], pa[6], dpa[6], zz[6], dz[6];
   T graw = T(0), vv = T(0), acc0 = T(0);
 #pragma unroll
-  for (int k = 0; k < 6; k++) o[k] = d[k] = T(0);
+  for (int k = 0; k < 6; k++) o[k] = d[k] = zz[k] = dz[k] = T(0);
   if (in) {
     int x, y, z;
     interior_coords(L, q, x, y, z);
@@ -260,7 +301,6 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
     long long nbr[6];
     bool fl[6];
     neighbours(L, x, y, z, nbr, fl);
-    T zz[6], dz[6];
 #pragma unroll
     for (int j = 0; j < 6; j++) {
       const long long nbi = nbr[j];
@@ -280,13 +320,15 @@ __global__ void __launch_bounds__(kB2) k_prop2b(DevLattice L, const T* __restric
         dz[5] = dn * dz[5] + (T(1) - dn) * d0;
       }
     }
-    transfer_rows(S, Lw, zz, o);
-    if constexpr (TANGENT) transfer_rows(S, Lw, dz, d);
     graw = g[r];
     if constexpr (TANGENT) vv = v[r];
     pair_rows<T, TANGENT, CURVED>(L, Wp, dWp, M, x, nbr, fl, r, pa, dpa);
     acc0 = acc[r];
   }
+  // outside the per-site branch: lanes without a site transfer zeros (the
+  // matrix-core form needs the whole wave)
+  transfer<T, MFMA>(S, Lw, zz, o);
+  if constexpr (TANGENT) transfer<T, MFMA>(S, Lw, dz, d);
   T scale = T(1), cfac = T(1), pscale = T(1);
   double lncum = 0;
   if constexpr (SCALED) {
@@ -497,15 +539,28 @@ __global__ void k_wsum6(DevLattice L, const T* __restrict__ G, const T* __restri
     }                                           \
   } while (0)
 
+// the matrix-core transfer runs for FP32 sweeps when the lattice asks for it
+// (sfx_options.transfer_kernel: 1 always, 2 in tangent sweeps only, where it
+// measured faster than the rows at 128^3; the base sweeps measured the other
+// way); FP64 keeps the structured rows
+static bool use_mfma(const DevLattice& L, bool tangent, size_t word) {
+  return word == 4 && (L.transfer_mfma == 1 || (L.transfer_mfma == 2 && tangent));
+}
 template <typename T>
 void launch_prop2f(const DevLattice& L, bool tangent, const T* Win, const T* dWin, const T* g, const T* v, T* Wout,
                    T* dWout, double S, const RankScale& sc, hipStream_t st) {
   const unsigned nb = grid1(L.V, kB2);
   const bool sc_on = sc.maxv != nullptr;
   const T s = (T)S;
-#define K(SC, TG, CV) k_prop2f<T, SC, TG, CV><<<nb, kB2, 0, st>>>(L, Win, dWin, g, v, Wout, dWout, s, sc)
-  SFX_O2_DISPATCH(K);
+  if (use_mfma(L, tangent, sizeof(T))) {
+#define K(SC, TG, CV) k_prop2f<T, SC, TG, CV, true><<<nb, kB2, 0, st>>>(L, Win, dWin, g, v, Wout, dWout, s, sc)
+    SFX_O2_DISPATCH(K);
 #undef K
+  } else {
+#define K(SC, TG, CV) k_prop2f<T, SC, TG, CV, false><<<nb, kB2, 0, st>>>(L, Win, dWin, g, v, Wout, dWout, s, sc)
+    SFX_O2_DISPATCH(K);
+#undef K
+  }
 }
 
 template <typename T>
@@ -514,10 +569,17 @@ void launch_prop2b(const DevLattice& L, bool tangent, const T* Gin, const T* dGi
   const unsigned nb = grid1(L.V, kB2);
   const bool sc_on = sc.maxv != nullptr;
   const T s = (T)S;
+  if (use_mfma(L, tangent, sizeof(T))) {
 #define K(SC, TG, CV) \
-  k_prop2b<T, SC, TG, CV><<<nb, kB2, 0, st>>>(L, Gin, dGin, g, v, Gout, dGout, Wp, dWp, acc, s, sc)
-  SFX_O2_DISPATCH(K);
+  k_prop2b<T, SC, TG, CV, true><<<nb, kB2, 0, st>>>(L, Gin, dGin, g, v, Gout, dGout, Wp, dWp, acc, s, sc)
+    SFX_O2_DISPATCH(K);
 #undef K
+  } else {
+#define K(SC, TG, CV) \
+  k_prop2b<T, SC, TG, CV, false><<<nb, kB2, 0, st>>>(L, Gin, dGin, g, v, Gout, dGout, Wp, dWp, acc, s, sc)
+    SFX_O2_DISPATCH(K);
+#undef K
+  }
 }
 
 template <typename T>

@@ -139,6 +139,10 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
     schedule = opt ? opt->chain_schedule : SFX_SCHEDULE_AUTO;
     SFX_REQUIRE(schedule == SFX_SCHEDULE_AUTO || schedule == SFX_SCHEDULE_SEQUENTIAL, SFX_E_INVALID,
                 "unknown chain_schedule");
+    const int tk = opt ? opt->transfer_kernel : SFX_TRANSFER_AUTO;
+    SFX_REQUIRE(tk >= SFX_TRANSFER_AUTO && tk <= SFX_TRANSFER_MFMA, SFX_E_INVALID, "unknown transfer_kernel");
+    // AUTO: the matrix cores in the tangent sweeps, the structured rows in the base sweeps
+    lat.d.transfer_mfma = tk == SFX_TRANSFER_MFMA ? 1 : (tk == SFX_TRANSFER_ROWS ? 0 : 2);
   }
   chi.assign(sd.chi, sd.chi + ns * ns);
   for (int i = 0; i < ns; i++)

@@ -143,9 +143,11 @@ class System:
 
     _SWEEP = {"auto": _abi.SWEEP_AUTO, "per_rank": _abi.SWEEP_PER_RANK, "blocked": _abi.SWEEP_BLOCKED}
     _SCHEDULE = {"auto": _abi.SCHEDULE_AUTO, "sequential": _abi.SCHEDULE_SEQUENTIAL}
+    _TRANSFER = {"auto": _abi.TRANSFER_AUTO, "rows": _abi.TRANSFER_ROWS, "mfma": _abi.TRANSFER_MFMA}
 
     def __init__(self, lat: LatticeSpec, sysd: SystemSpec, precision: str = "fp64",
-                 memory_budget: int = 0, sweep_kernel: str = "auto", chain_schedule: str = "auto"):
+                 memory_budget: int = 0, sweep_kernel: str = "auto", chain_schedule: str = "auto",
+                 transfer_kernel: str = "auto"):
         self.lat, self.sys = lat, sysd
         self._ld = lat.desc()
         self._sd, self._keep = sysd.desc(lat)
@@ -154,6 +156,7 @@ class System:
         opt.memory_budget = memory_budget
         opt.sweep_kernel = self._SWEEP[sweep_kernel]
         opt.chain_schedule = self._SCHEDULE[chain_schedule]
+        opt.transfer_kernel = self._TRANSFER[transfer_kernel]
         self.precision = opt.precision
         h = C.c_void_p()
         check(lib().sfx_system_create(C.byref(self._ld), C.byref(self._sd), C.byref(opt), C.byref(h)))

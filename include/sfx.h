@@ -130,10 +130,21 @@ typedef struct sfx_options {
                              (SF_Copol1stO.cpp:1186-1281). The two schedules agree to
                              rounding; a checkpointed store repeats the sequential one
                              bit for bit. No counterpart in the reference. */
+  int transfer_kernel;    /* second order, FP32 sweeps: how the 6x6 direction transfer
+                             (Lat3D2ndO.cpp:64-152, 597-678) runs at every site:
+                             SFX_TRANSFER_ROWS, its structured form S in_k +
+                             L (sum - in_k - in_5-k) on the vector units;
+                             SFX_TRANSFER_MFMA, the dense 6x6 product on the matrix
+                             cores (v_mfma_f32_4x4x1f32, each lane keeping its site);
+                             SFX_TRANSFER_AUTO, the matrix cores in the tangent sweeps
+                             and the rows in the base sweeps (each the faster of the two
+                             there on MI355X). The two agree to FP32 rounding; FP64
+                             sweeps use the rows. No counterpart in the reference. */
 } sfx_options;
 
 enum { SFX_SWEEP_AUTO = 0, SFX_SWEEP_PER_RANK = 1, SFX_SWEEP_BLOCKED = 2 };
 enum { SFX_SCHEDULE_AUTO = 0, SFX_SCHEDULE_SEQUENTIAL = 1 };
+enum { SFX_TRANSFER_AUTO = 0, SFX_TRANSFER_ROWS = 1, SFX_TRANSFER_MFMA = 2 };
 
 typedef struct sfx_system sfx_system;
 

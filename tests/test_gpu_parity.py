@@ -421,3 +421,25 @@ def test_grand_and_chemical_potentials(built, name, sfx):
     for m in range(len(sysd.molecules)):
         mg, mo = G.chemical_potential(m), O.chemical_potential(m)
         assert abs(mg - mo) <= 1e-10 * max(1.0, abs(mo)), (m, mg, mo)
+
+
+@pytest.mark.parametrize("name", ["3d_sc_order2_homopolymer", "3d_sc_order2_copolymer", "1d_order2_adsorption",
+                                  "1d_cyl_order2", "2d_cyl_order2"])
+def test_second_order_transfer_on_matrix_cores(built, name, sfx):
+    """The 6x6 direction transfer on the matrix cores (v_mfma_f32_4x4x1f32)
+    against the structured rows: FP32 residuals and JVPs agree to FP32
+    rounding, and both stay within the FP32 bound of the FP64 oracle."""
+    lat, sysd, G, O = built[name]
+    Gm = sfx.System(lat, sysd, transfer_kernel="mfma")
+    Gr = sfx.System(lat, sysd, transfer_kernel="rows")
+    x = cases.start(lat, sysd, seed=21)
+    fo = O.residual(x)
+    fm, fr = Gm.residual(x, precision="fp32"), Gr.residual(x, precision="fp32")
+    print(f"{name}: |f_mfma - f_rows| = {np.abs(fm - fr).max():.2e}, |f_mfma - f_oracle| = {np.abs(fm - fo).max():.2e}")
+    assert np.abs(fm - fr).max() < 1e-5
+    assert np.abs(fm - fo).max() < 2e-4
+    v = np.random.default_rng(3).normal(0, 1, G.nvar)
+    jm, jr = Gm.jvp(x, v, precision="fp32"), Gr.jvp(x, v, precision="fp32")
+    assert np.abs(jm - jr).max() <= 1e-4 * np.abs(jr).max()
+    # FP64 sweeps keep the rows: the option changes nothing there
+    np.testing.assert_array_equal(Gm.residual(x, precision="fp64"), Gr.residual(x, precision="fp64"))
