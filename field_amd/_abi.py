@@ -16,7 +16,7 @@ FREE, RESTRICTED, SOLVENT = 0, 1, 2
 FORM_SFBOX, FORM_MELT = 0, 1
 FP64, FP32 = 0, 1
 HOST, DEVICE = 0, 1
-SOLVER_DIIS, SOLVER_NK = 0, 1
+SOLVER_DIIS, SOLVER_NK, SOLVER_NEWTON = 0, 1, 2
 
 _BOUNDS = {"surface": SURFACE, "mirror1": MIRROR1, "mirror2": MIRROR2, "periodic": PERIODIC}
 _GEOM = {"flat": FLAT, "cylindrical": CYLINDER, "cylinder": CYLINDER, "spherical": SPHERE,
@@ -90,6 +90,9 @@ class SolverOpts(C.Structure):
         ("ptc_dt0", C.c_double),
         ("verbose", C.c_int),
         ("diis_backtrack", C.c_int),
+        ("newton_deltamax", C.c_double),
+        ("newton_linesearch_limit", C.c_int),
+        ("newton_reset_criterion", C.c_double),
     ]
 
 
@@ -113,6 +116,8 @@ class SolveStats(C.Structure):
         ("residual_norm", C.c_double),
         ("seconds", C.c_double),
         ("converged", C.c_int),
+        ("accuracy", C.c_double),
+        ("graph_replays", C.c_int),
     ]
 
 

@@ -1,12 +1,14 @@
 // sfx_capi.hip -- the extern "C" boundary (include/sfx.h). No exception
 // crosses it: every entry point converts failures into a status code and a
 // thread-local message.
+#include <chrono>
 #include <cstdarg>
 #include <cstring>
 #include <new>
 #include <string>
 
 #include "../../include/sfx_profile.h"
+#include "sfx_newton.h"
 #include "sfx_solve.h"
 
 namespace sfx {
@@ -214,6 +216,9 @@ int sfx_solver_defaults(sfx_solver_opts* o) {
   o->ptc_dt0 = 1.0;
   o->verbose = 0;
   o->diis_backtrack = 0;
+  o->newton_deltamax = 0.5;          // SF_Solve.cpp:1888 (Scheutjens line search)
+  o->newton_linesearch_limit = 20;   // sfnewton.cpp:31
+  o->newton_reset_criterion = 1e3;   // SF_Solve.cpp:2176
   return SFX_OK;
 }
 
@@ -238,15 +243,40 @@ int sfx_solve(sfx_system* s, const sfx_solver_opts* o, double* x, int where, sfx
     System& S = s->s;
     sfx_solve_stats st{};
     S.load_x(x, where);
+    const long long replays0 = S.graph_replays;
     if (o->method == SFX_SOLVER_DIIS)
       solve_diis(S, *o, st);
     else if (o->method == SFX_SOLVER_NK)
       solve_nk(S, *o, st);
+    else if (o->method == SFX_SOLVER_NEWTON)
+      solve_newton(S, *o, st);
     else
       throw Error{SFX_E_INVALID, "unknown solver method"};
     S.store_f(S.d_u, x, where);
+    st.graph_replays = (int)(S.graph_replays - replays0);
     if (stats) *stats = st;
     if (!st.converged) throw Error{SFX_E_NOTCONVERGED, "solver reached its iteration limit"};
+  });
+}
+
+int sfx_newton_iterate(sfx_residual_fn f, void* ctx, double* x, long n, const sfx_solver_opts* o,
+                       sfx_solve_stats* stats) {
+  return guarded([&] {
+    SFX_REQUIRE(f && x && o, SFX_E_INVALID, "NULL argument");
+    SFX_REQUIRE(n >= 2 && n <= kNewtonMaxVars, SFX_E_INVALID, "n must be in 2..16384");
+    const NewtonParams P = newton_params(*o);
+    const auto t0 = std::chrono::steady_clock::now();
+    NewtonResult R;
+    try {
+      R = pseudo_newton([&](const double* xv, double* g) { f(ctx, xv, g); }, x, (int)n, P);
+    } catch (const std::bad_alloc&) {
+      throw Error{SFX_E_OOM, "no host memory for the pseudo-Hessian"};
+    }
+    sfx_solve_stats st{};
+    fill_stats(R, st);
+    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    if (!st.converged) throw Error{SFX_E_NOTCONVERGED, "Newton stopped before the tolerance"};
   });
 }
 

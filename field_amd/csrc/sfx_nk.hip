@@ -71,7 +71,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
 
   int prec = o.fp64_switch > 0 ? SFX_FP32 : SFX_FP64;
   auto eval = [&](int p) {
-    S.residual_internal(p);
+    S.residual_replay(p);
     if (p == SFX_FP32)
       st.residual_evals_fp32++;
     else
@@ -282,7 +282,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
   st.seconds = ms * 1e-3;
   st.converged = converged;
   if (prec != SFX_FP64) {  // report the FP64 residual of the returned point
-    S.residual_internal(SFX_FP64);
+    S.residual_replay(SFX_FP64);
     st.residual_norm = std::sqrt(V.dot(S.d_f, S.d_f));
   }
 }

@@ -78,5 +78,11 @@ SolveWork& solve_work(System& S, long long n, size_t nvec);
 void diis_coeffs(const std::vector<double>& A, int n, std::vector<double>& X);
 void solve_diis(System& S, const sfx_solver_opts& o, sfx_solve_stats& st);
 void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st);
+constexpr long long kNewtonMaxVars = 16384;  // a 1 GiB float pseudo-Hessian
+void solve_newton(System& S, const sfx_solver_opts& o, sfx_solve_stats& st);
+struct NewtonParams;
+struct NewtonResult;
+NewtonParams newton_params(const sfx_solver_opts& o);
+void fill_stats(const NewtonResult& R, sfx_solve_stats& st);
 
 }  // namespace sfx

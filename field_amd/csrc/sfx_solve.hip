@@ -242,7 +242,7 @@ void solve_diis(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
   std::vector<double> Aij(m * m, 0.0), Apij, Ci;
   int it = 0, k_diis = 1, k = 0;
   V.copy(x, x0);  // remember the starting point for x_x0 = x - x0 (x0 = 0 in MPS.cpp:264)
-  S.residual_internal(SFX_FP64);
+  S.residual_replay(SFX_FP64);
   st.residual_evals++;
   V.axpby(-eta, g, 1.0, x);
   V.sub(x, x0, x_x0(0));
@@ -252,7 +252,7 @@ void solve_diis(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
   while (err > o.tolerance && it < o.max_iter) {
     it++;
     V.copy(x, x0);
-    S.residual_internal(SFX_FP64);
+    S.residual_replay(SFX_FP64);
     st.residual_evals++;
     k = it % m;
     k_diis++;
@@ -310,7 +310,7 @@ void solve_diis(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
   // the residual of the returned x (leaving phi / free energy consistent
   // with it): at the iteration limit x is the last, untested DIIS
   // combination, so the norm and the verdict come from this evaluation
-  S.residual_internal(SFX_FP64);
+  S.residual_replay(SFX_FP64);
   const double fin = std::sqrt(V.dot(g, g));
   st.iterations = it;
   st.residual_norm = fin;

@@ -202,6 +202,16 @@ struct System {
   void abi_vectors(double*& in, double*& out);
   void compute_phi(int precision);  // from d_u
   void residual_internal(int precision);  // d_u -> d_f
+  // residual_internal replayed from a HIP graph captured on the first call
+  // (per precision): one submission instead of a launch per rank. Plain
+  // launches while kernel timing is on. Kernels take their parameters by
+  // value, so set_chi drops the graphs.
+  void residual_replay(int precision);
+  void drop_graphs();
+  hipGraphExec_t res_graph[2] = {nullptr, nullptr};
+  bool res_graph_gmax[2] = {false, false};
+  bool graph_ok = true;  // false once a capture was refused: plain launches from then on
+  long long graph_replays = 0;  // residuals replayed from a graph so far
   double free_energy();
   double grand_potential();
   double chemical_potential(int m);

@@ -62,6 +62,7 @@ static T* dalloc(size_t n) {
 }
 
 System::~System() {
+  drop_graphs();
   void* ptrs[] = {d_u, d_f, d_swf, d_swf32, d_phi_m, d_acc, d_acc2, d_phi, d_rho, d_phitot, d_fprof, d_frozen,
                   d_st_itvar, d_mol_uses, d_itmask, d_sites, d_store, d_pp, d_pp2, d_molscal, d_lnf, d_lnb,
                   d_maxf, d_maxb, d_partial, d_phib_state, d_cbulk, d_scalar, d_xabi, d_gmax,
@@ -116,6 +117,7 @@ void System::set_chi(int a, int b, double value) {
     sp.mol_ref[m] = mols[m].chem_int_ref;
   }
   precond_update(precond, *this);  // the preconditioner is the Jacobian at this chi
+  drop_graphs();
   phi_valid = false;
 }
 
@@ -846,6 +848,58 @@ void System::residual_internal(int prec) {
   compute_phi(prec);
   launch_residual(lat.d, sp, d_itmask, d_swf, d_rho, d_phitot, d_cbulk, d_f, stream);
   SFX_HIP(hipGetLastError());
+}
+
+void System::drop_graphs() {
+  for (auto& g : res_graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+void System::residual_replay(int prec) {
+  const int k = prec == SFX_FP32 ? 1 : 0;
+  if (prof.on || !graph_ok) {
+    residual_internal(prec);
+    return;
+  }
+  if (!res_graph[k]) {
+    // the first call runs its launches (which also makes one-time launch
+    // settings such as dynamic LDS attributes outside any capture), then
+    // records them for the calls after it: a capture executes nothing. A
+    // runtime that refuses the capture leaves the system on plain launches.
+    residual_internal(prec);
+    const bool gm = gmax_valid;
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipGetLastError();
+      graph_ok = false;
+      return;
+    }
+    bool captured = true;
+    try {
+      residual_internal(prec);
+    } catch (...) {
+      captured = false;
+    }
+    if (hipStreamEndCapture(stream, &g) != hipSuccess || !captured || !g ||
+        hipGraphInstantiate(&res_graph[k], g, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      res_graph[k] = nullptr;
+      graph_ok = false;
+    }
+    if (g) (void)hipGraphDestroy(g);
+    res_graph_gmax[k] = gmax_valid = gm;
+    phi_valid = true;
+    last_precision = prec;
+    return;
+  }
+  SFX_HIP(hipGraphLaunch(res_graph[k], stream));
+  graph_replays++;
+  phi_valid = true;
+  last_precision = prec;
+  gmax_valid = res_graph_gmax[k];
 }
 
 double System::free_energy() {

@@ -42,6 +42,7 @@ _SIGS = {
     "sfx_stream_set": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sfx_solver_defaults": (C.c_int, [C.c_void_p]),
     "sfx_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "sfx_newton_iterate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_void_p, C.c_void_p]),
     "sfx_precondition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "sfx_profile_start": (C.c_int, [C.c_void_p]),
     "sfx_profile_start_select": (C.c_int, [C.c_void_p, C.c_uint]),
@@ -136,6 +137,34 @@ def _ptr(a):
 
 
 _PREC = {"fp64": _abi.FP64, "fp32": _abi.FP32, None: None}
+
+
+RESIDUAL_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def newton_iterate(fun, x0, check_converged=True, **kw):
+    """SFNewton's iteration (sfx_newton_iterate) on g = fun(x), host only:
+    returns (x, stats). kw sets sfx_solver_opts fields (tolerance, max_iter,
+    newton_deltamax, newton_linesearch_limit, newton_reset_criterion)."""
+    o = _abi.SolverOpts()
+    check(lib().sfx_solver_defaults(C.byref(o)))
+    o.method = _abi.SOLVER_NEWTON
+    for k, v in kw.items():
+        setattr(o, k, v)
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    n = x.size
+
+    def cb(_ctx, xp, gp):
+        xv = np.ctypeslib.as_array(xp, shape=(n,)).copy()
+        np.ctypeslib.as_array(gp, shape=(n,))[:] = fun(xv)
+
+    fn = RESIDUAL_FN(cb)
+    st = _abi.SolveStats()
+    rc = lib().sfx_newton_iterate(fn, None, x.ctypes.data_as(C.c_void_p), n, C.byref(o), C.byref(st))
+    stats = {k: getattr(st, k) for k, _ in _abi.SolveStats._fields_}
+    if rc != 0 and not (rc == _abi.SFX_E_NOTCONVERGED and not check_converged):
+        raise SFError(rc, last_error())
+    return x, stats
 
 
 class System:
@@ -295,7 +324,7 @@ class System:
     def solve(self, x0, method="diis", check_converged=True, **kw):
         o = _abi.SolverOpts()
         check(lib().sfx_solver_defaults(C.byref(o)))
-        o.method = {"diis": _abi.SOLVER_DIIS, "nk": _abi.SOLVER_NK}[method]
+        o.method = {"diis": _abi.SOLVER_DIIS, "nk": _abi.SOLVER_NK, "newton": _abi.SOLVER_NEWTON}[method]
         for k, v in kw.items():
             if k == "jvp_precision":
                 v = _PREC[v]

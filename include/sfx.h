@@ -214,7 +214,15 @@ int sfx_set_chi(sfx_system* s, int a, int b, double value);
 int sfx_stream_set(sfx_system* s, void* stream);
 
 /* ---- solvers ----------------------------------------------------------- */
-enum { SFX_SOLVER_DIIS = 0, SFX_SOLVER_NK = 1 };
+/* SFX_SOLVER_NEWTON: the reference's default iteration, SFNewton with a dense
+ * pseudo-Hessian (a float L D U factorisation on the host, one rank-one update
+ * per step) and the Scheutjens line search and trust region
+ * (sfnewton.cpp:844-1143,1433-1530,1916-2056; newttool.cpp), with the reset
+ * rules of SF_Solve::inneriteration (SF_Solve.cpp:659-760). Every residual is
+ * FP64 on the GPU. It stops as the reference does: accuracy
+ * sqrt(|p| |g| / (1 + |x|)) <= tolerance and |g| <= 10 tolerance. At most
+ * 16384 iteration variables (SFX_E_UNSUPPORTED beyond). */
+enum { SFX_SOLVER_DIIS = 0, SFX_SOLVER_NK = 1, SFX_SOLVER_NEWTON = 2 };
 
 typedef struct sfx_solver_opts {
   int method;             /* SFX_SOLVER_DIIS (MPS.cpp:228-284) or SFX_SOLVER_NK */
@@ -246,6 +254,11 @@ typedef struct sfx_solver_opts {
                              history and halves that problem's step (giving up below 1e-4 of
                              diis_eta, at the best point); 0 = the plain iteration of
                              MPS.cpp:228-284. No counterpart in the reference. */
+  double newton_deltamax;       /* Newton: largest step length (SF_Solve.cpp:1888: 0.5) */
+  int newton_linesearch_limit;  /* Newton: evaluations per line search (sfnewton.cpp:31: 20) */
+  double newton_reset_criterion; /* Newton: reset the pseudo-Hessian when the accuracy
+                                    exceeds this multiple of the best so far
+                                    (SF_Solve.cpp:2176: 1e3) */
 } sfx_solver_opts;
 
 typedef struct sfx_solve_stats {
@@ -256,6 +269,11 @@ typedef struct sfx_solve_stats {
   double residual_norm;   /* final ||f||_2 of the FP64 residual */
   double seconds;         /* wall time of the solve (HIP events) */
   int converged;
+  double accuracy;        /* Newton: the reference's accuracy sqrt(|p| |g| / (1 + |x|))
+                             at the end (sfnewton.cpp:2040); 0 for the other methods */
+  int graph_replays;      /* residual evaluations replayed from the system's captured
+                             HIP graph (the others ran as separate launches: the first
+                             per precision, and all of them while kernel timing is on) */
 } sfx_solve_stats;
 
 int sfx_solver_defaults(sfx_solver_opts* o);
@@ -266,6 +284,17 @@ int sfx_precondition(sfx_system* s, const double* r, double* out, int where);
 /* Solves f(x)=0 from x (in/out, host or device per `where`). */
 int sfx_solve(sfx_system* s, const sfx_solver_opts* o, double* x, int where,
               sfx_solve_stats* stats);
+
+/* SFNewton::iterate(x, n) (sfnewton.h:87, sfnewton.cpp:844) on a residual
+ * the caller supplies, as a subclass's override of SFNewton::residuals(g, x)
+ * (sfnewton.h:84) supplies it: the same iteration as SFX_SOLVER_NEWTON, on
+ * host memory, with no GPU work of its own. Reads tolerance, max_iter and the
+ * newton_* fields of o; stats gets iterations, residual_evals, residual_norm,
+ * accuracy, converged and the wall time. SFX_E_NOTCONVERGED if it stopped at
+ * max_iter or on a residual that is not a number (x holds the last point). */
+typedef void (*sfx_residual_fn)(void* ctx, const double* x, double* g);
+int sfx_newton_iterate(sfx_residual_fn f, void* ctx, double* x, long n, const sfx_solver_opts* o,
+                       sfx_solve_stats* stats);
 
 /* ---- batched 1-D adsorption solves --------------------------------------
  * Replaces running a Vary sweep's calculations one after the other

@@ -90,6 +90,31 @@ int orc_mps_scf(orc_mps* p, double* x, double tol, int max_iter, int m, double e
 /* small dense symmetric solve used by DIIS: X = A^{-1} 1 (MPS.cpp:200-226) */
 void orc_diis_coeffs(const double* A, int n, double* X);
 
+/* SFNewton's pseudo-Hessian Newton with the Scheutjens line search
+ * (orc_newton.c). The factorisation routines work on the reference's float
+ * n x n L D U array (element (r, c) at h[r + n c]). */
+typedef void (*orc_resid_fn)(void* ctx, const double* x, double* g);
+typedef struct orc_newton_opts {
+  double tolerance;
+  int iterationlimit;
+  double deltamax, deltamin;
+  int linesearchlimit;
+  double reset_hessian_criterion;
+  double small_alpha;
+  int max_n_small_alpha;
+} orc_newton_opts;
+typedef struct orc_newton_stats {
+  int iterations, functioncalls, hessian_resets, reversed_directions, time_dependent, converged;
+  double accuracy, residual_norm;
+} orc_newton_stats;
+void orc_newton_defaults(orc_newton_opts* o);
+int orc_newton(orc_resid_fn f, void* ctx, double* x, int n, const orc_newton_opts* o, orc_newton_stats* st);
+int orc_newton_sys(orc_sys* s, double* x, int n, const orc_newton_opts* o, orc_newton_stats* st);
+void orc_ldu_multiply(const float* h, int n, const double* w, double alpha, double* v);
+int orc_ldu_sign(const float* h, int n);
+void orc_ldu_update(float* h, int n, double* w, double* v, double alpha);
+void orc_ldu_solve(const float* h, int n, const double* g, double* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,8 @@ _lib = None
 D = C.c_double
 PD = C.POINTER(C.c_double)
 PI = C.POINTER(C.c_int)
+PF = C.POINTER(C.c_float)
+RESID_FN = C.CFUNCTYPE(None, C.c_void_p, PD, PD)
 
 
 def build():
@@ -87,12 +89,102 @@ def lib():
         L.orc_propagate_force.argtypes = [C.c_void_p, PD, PD, PD, D]
         L.orc_norm_phi_free.argtypes = [C.c_void_p, PD, PD, D]
         L.orc_norm_phi_restr.argtypes = [C.c_void_p, PD, PD, PD, D]
+        L.orc_newton_defaults.argtypes = [C.c_void_p]
+        L.orc_newton.argtypes = [RESID_FN, C.c_void_p, PD, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_newton_sys.argtypes = [C.c_void_p, PD, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_ldu_multiply.argtypes = [PF, C.c_int, PD, D, PD]
+        L.orc_ldu_sign.argtypes = [PF, C.c_int]
+        L.orc_ldu_update.argtypes = [PF, C.c_int, PD, PD, D]
+        L.orc_ldu_solve.argtypes = [PF, C.c_int, PD, PD]
         _lib = L
     return _lib
 
 
 def _p(a):
     return a.ctypes.data_as(PD)
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("iterationlimit", C.c_int), ("deltamax", C.c_double),
+                ("deltamin", C.c_double), ("linesearchlimit", C.c_int),
+                ("reset_hessian_criterion", C.c_double), ("small_alpha", C.c_double),
+                ("max_n_small_alpha", C.c_int)]
+
+
+class NewtonStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("functioncalls", C.c_int), ("hessian_resets", C.c_int),
+                ("reversed_directions", C.c_int), ("time_dependent", C.c_int), ("converged", C.c_int),
+                ("accuracy", C.c_double), ("residual_norm", C.c_double)]
+
+
+def _newton_opts(**kw):
+    o = NewtonOpts()
+    lib().orc_newton_defaults(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _stats_dict(st):
+    return {k: getattr(st, k) for k, _ in NewtonStats._fields_}
+
+
+def newton(fun, x, **kw):
+    """SFNewton (pseudo-Hessian Newton, Scheutjens line search) on g = fun(x);
+    returns (x, stats)."""
+    x = _f64(x).copy()
+    n = x.size
+
+    def cb(_ctx, xp, gp):
+        xv = np.ctypeslib.as_array(xp, shape=(n,))
+        np.ctypeslib.as_array(gp, shape=(n,))[:] = fun(xv.copy())
+
+    fn = RESID_FN(cb)
+    st = NewtonStats()
+    rc = lib().orc_newton(fn, None, _p(x), n, C.byref(_newton_opts(**kw)), C.byref(st))
+    if rc:
+        raise ValueError(f"orc_newton returned {rc}")
+    return x, _stats_dict(st)
+
+
+def ldu_identity(n, diag=1.0):
+    h = np.zeros((n, n), dtype=np.float32)
+    np.fill_diagonal(h, diag)
+    return h
+
+
+def ldu_dense(h):
+    """The matrix L D U that a factorisation array h stands for (h[r + n c] is
+    element (r, c): numpy's column-major view, h.T in row-major terms)."""
+    a = np.asarray(h, dtype=np.float64).T
+    L = np.tril(a, -1) + np.eye(a.shape[0])
+    U = np.triu(a, 1) + np.eye(a.shape[0])
+    return L @ np.diag(np.diag(a)) @ U
+
+
+def _pf(a):
+    return a.ctypes.data_as(PF)
+
+
+def ldu_update(h, w, v, alpha):
+    w, v = _f64(w).copy(), _f64(v).copy()
+    lib().orc_ldu_update(_pf(h), h.shape[0], _p(w), _p(v), alpha)
+
+
+def ldu_solve(h, g):
+    p = np.zeros(h.shape[0])
+    lib().orc_ldu_solve(_pf(h), h.shape[0], _p(_f64(g)), _p(p))
+    return p
+
+
+def ldu_multiply(h, w, alpha=1.0):
+    v = np.zeros(h.shape[0])
+    lib().orc_ldu_multiply(_pf(h), h.shape[0], _p(_f64(w)), alpha, _p(v))
+    return v
+
+
+def ldu_sign(h):
+    return lib().orc_ldu_sign(_pf(h), h.shape[0])
 
 
 def _f64(a):
@@ -228,6 +320,15 @@ class System:
 
     def grand_potential(self):
         return lib().orc_sys_grand_potential(self.h)
+
+    def solve_newton(self, x, **kw):
+        """SFNewton's default iteration from x; returns (x, stats)."""
+        x = _f64(x).copy()
+        st = NewtonStats()
+        rc = lib().orc_newton_sys(self.h, _p(x), x.size, C.byref(_newton_opts(**kw)), C.byref(st))
+        if rc:
+            raise ValueError(f"orc_newton_sys returned {rc}")
+        return x, _stats_dict(st)
 
     def solve_diis(self, x, tol=1e-7, max_iter=1000, m=10, eta=0.1):
         x = _f64(x).copy()
