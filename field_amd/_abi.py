@@ -12,7 +12,7 @@ SFX_E_NOTCONVERGED = -6
 FLAT, CYLINDER, SPHERE = 0, 1, 2
 LAT_STANDARD, LAT_FCC, LAT_HEX = 0, 1, 2
 SURFACE, MIRROR1, MIRROR2, PERIODIC = 1, 2, 3, 4
-FREE, RESTRICTED, SOLVENT = 0, 1, 2
+FREE, RESTRICTED, SOLVENT, RANGE_RESTRICTED = 0, 1, 2, 3
 FORM_SFBOX, FORM_MELT = 0, 1
 FP64, FP32 = 0, 1
 HOST, DEVICE = 0, 1
@@ -47,6 +47,9 @@ class MoleculeDesc(C.Structure):
         ("freedom", C.c_int),
         ("phibulk", C.c_double),
         ("theta", C.c_double),
+        ("range_lo", C.c_int * 3),
+        ("range_hi", C.c_int * 3),
+        ("phi_range", C.c_double),
     ]
 
 

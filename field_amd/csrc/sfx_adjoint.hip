@@ -947,6 +947,8 @@ void tangent_free(Tangent* t) { delete t; }
 
 // J v at the base state of the last compute_phi / residual at S.d_u.
 void jvp_at_base(System& S, const double* v, double* jv, int precision) {
+  SFX_REQUIRE(!S.has_range(), SFX_E_UNSUPPORTED,
+              "Jacobian products of range_restricted molecules are not built; solve with DIIS or Newton");
   if (!S.tan) {
     S.tan = new Tangent();
     S.tan->init(S);
@@ -1214,6 +1216,8 @@ static double host_sum(const double* part, int n, hipStream_t st) {
 
 // J^T w at the base state of the last compute_phi at S.d_u
 void vjp_at_base(System& S, const double* w, double* jtw, int precision) {
+  SFX_REQUIRE(!S.has_range(), SFX_E_UNSUPPORTED,
+              "Jacobian products of range_restricted molecules are not built; solve with DIIS or Newton");
   SFX_REQUIRE(vjp_supported(S), SFX_E_UNSUPPORTED,
               "transposed products need a backward sweep that is the forward sweep's adjoint: not on a curved "
               "lattice with a mirror2 side, the second-order 2-D cylinder, a layer adjustment other than 1, a "

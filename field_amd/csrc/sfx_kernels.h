@@ -9,7 +9,7 @@
 namespace sfx {
 
 // per-molecule device scalars
-enum { MS_LNGN = 0, MS_LNC, MS_PHIB, MS_PREF, MS_REF, MS_COUNT = 8 };
+enum { MS_LNGN = 0, MS_LNC, MS_PHIB, MS_PREF, MS_REF, MS_RANGE_C, MS_COUNT = 8 };
 
 // FP32 per-rank log-scaling context of one sweep. Step k writes max|G| of its
 // output into maxv[k] and the cumulative log scale into lnv[k]; its output is
@@ -153,5 +153,17 @@ int launch_grand_potential(const DevLattice& L, const SysParams& sp, const doubl
                            const double* rho, const double* swf, const double* molscal, const double* phib_state,
                            const double* cbulk, double* partial, hipStream_t st);
 void launch_sum_partials(const double* partial, int n, double* out, hipStream_t st);
+
+// range_restricted molecules (SF_Copol1stO.cpp:1246-1280): phi_m of the used
+// states summed over the interior sites of the box lo..hi (partial sums per
+// block), then C = phi_range / sum into the molecule's scalars (pref, ln C,
+// phi_b = N C) and phi_m *= C
+struct RangeBox {
+  int lo[3], hi[3];
+};
+int launch_range_sum(const DevLattice& L, RangeBox box, unsigned uses, int ns, const double* phi_m, double* partial,
+                     hipStream_t st);
+void launch_range_scale(const DevLattice& L, unsigned uses, int ns, const double* partial, int nparts,
+                        double phi_range, int N, double* scal, double* phi_m, hipStream_t st);
 
 }  // namespace sfx

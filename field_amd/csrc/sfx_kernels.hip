@@ -229,7 +229,12 @@ __global__ void k_mol_finalize(MolParams mp, const double* __restrict__ partial,
   // half-stored chains (SymMatrix1 factor 2); second order divides by 6 (Lat3D2ndO.cpp:1037)
   double f = mp.kind == 2 ? 2.0 : mp.kind == 3 ? 1.0 / 6.0 : 1.0;
   double pref;
-  if (mp.freedom == SFX_RESTRICTED) {
+  if (mp.freedom == SFX_RANGE_RESTRICTED) {
+    // the profile before the range condition fixes its constant (k_range_finalize)
+    lnC = 0.0;
+    phib = 0.0;
+    pref = f * exp(ref);
+  } else if (mp.freedom == SFX_RESTRICTED) {
     lnC = log(mp.theta) - lnGN - (mp.kind == 0 ? 0.0 : log(N));
     phib = mp.theta / exp(lnGN);
     lnpref = lnC;
@@ -714,6 +719,70 @@ template void launch_connect<float, float>(const DevLattice&, const float*, cons
                                            double, hipStream_t);
 template void launch_copy_interior<double>(const DevLattice&, const double*, double*, hipStream_t);
 template void launch_copy_interior<float>(const DevLattice&, const float*, float*, hipStream_t);
+__global__ void k_range_sum(DevLattice L, RangeBox box, unsigned uses, int ns, const double* __restrict__ phi_m,
+                            double* __restrict__ partial) {
+  const long long q = site_ordinal();
+  double v = 0;
+  if (q < L.V) {
+    int x, y, z;
+    interior_coords(L, q, x, y, z);
+    const int c[3] = {x, y, z};
+    bool in = true;
+    for (int a = 0; a < L.grad; a++) in = in && c[a] >= box.lo[a] && c[a] <= box.hi[a];
+    if (in) {
+      const long long r = sidx(L, x, y, z);
+      for (int t = 0; t < ns; t++)
+        if (uses >> t & 1u) v += phi_m[(size_t)t * L.M + r];
+    }
+  }
+  v = block_sum(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+__global__ void k_range_finalize(const double* __restrict__ partial, int nparts, double phi_range, int N,
+                                 double* __restrict__ scal) {
+  __shared__ double red[256 / 64];
+  double s = 0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += partial[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  s = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i];
+  const double C = phi_range / s;
+  scal[MS_RANGE_C] = C;
+  scal[MS_PREF] *= C;
+  scal[MS_LNC] = log(C);
+  // theta = sum_z L phi = C N G_N (every rank composes to G_N), so phi_b = theta / G_N = N C
+  scal[MS_PHIB] = (double)N * C;
+}
+
+__global__ void k_range_scale(DevLattice L, unsigned uses, int ns, const double* __restrict__ scal,
+                              double* __restrict__ phi_m) {
+  const long long q = site_ordinal();
+  if (q >= L.V) return;
+  int x, y, z;
+  interior_coords(L, q, x, y, z);
+  const long long r = sidx(L, x, y, z);
+  const double C = scal[MS_RANGE_C];
+  for (int t = 0; t < ns; t++)
+    if (uses >> t & 1u) phi_m[(size_t)t * L.M + r] *= C;
+}
+
+int launch_range_sum(const DevLattice& L, RangeBox box, unsigned uses, int ns, const double* phi_m, double* partial,
+                     hipStream_t st) {
+  const unsigned nb = grid1(L.V, kBlock);
+  k_range_sum<<<nb, kBlock, 0, st>>>(L, box, uses, ns, phi_m, partial);
+  return (int)nb;
+}
+
+void launch_range_scale(const DevLattice& L, unsigned uses, int ns, const double* partial, int nparts,
+                        double phi_range, int N, double* scal, double* phi_m, hipStream_t st) {
+  k_range_finalize<<<1, 256, 0, st>>>(partial, nparts, phi_range, N, scal);
+  k_range_scale<<<grid1(L.V, kBlock), kBlock, 0, st>>>(L, uses, ns, scal, phi_m);
+}
+
 template int launch_wsum<double>(const DevLattice&, const double*, double*, hipStream_t);
 template int launch_wsum<float>(const DevLattice&, const float*, double*, hipStream_t);
 template void launch_mol_phi<double>(const DevLattice&, const double*, const double*, const double*, int, double*,

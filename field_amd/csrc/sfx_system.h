@@ -87,6 +87,8 @@ struct Molecule {
   std::vector<int> seq;
   std::vector<int> uses;  // per state: 1 if the chain contains it
   double phib = 0, theta = 0, chem_int_ref = 0;
+  RangeBox range{};       // range_restricted: the box and the amount in it
+  double phi_range = 0;
   int nstore = 0;         // forward ranks the backward pass needs
   ChainPlan plan;         // how they are kept
   MolParams params(int index) const {
@@ -202,6 +204,8 @@ struct System {
   void abi_vectors(double*& in, double*& out);
   void compute_phi(int precision);  // from d_u
   void residual_internal(int precision);  // d_u -> d_f
+  void finish_range(int m);  // range_restricted: the range condition fixes molecule m's constant
+  bool has_range() const;
   // residual_internal replayed from a HIP graph captured on the first call
   // (per precision): one submission instead of a launch per rank. Plain
   // launches while kernel timing is on. Kernels take their parameters by

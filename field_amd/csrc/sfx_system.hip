@@ -220,6 +220,21 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
     if (d.freedom == SFX_SOLVENT) nsolv++;
     if (d.freedom == SFX_RESTRICTED) SFX_REQUIRE(d.theta > 0, SFX_E_INVALID, "restricted molecule needs theta > 0");
     if (d.freedom == SFX_FREE) SFX_REQUIRE(d.phibulk > 0, SFX_E_INVALID, "free molecule needs phibulk > 0");
+    SFX_REQUIRE(d.freedom >= SFX_FREE && d.freedom <= SFX_RANGE_RESTRICTED, SFX_E_INVALID, "unknown freedom");
+    if (d.freedom == SFX_RANGE_RESTRICTED) {
+      SFX_REQUIRE(form == SFX_FORM_SFBOX, SFX_E_INVALID, "range_restricted needs the sfbox form");
+      SFX_REQUIRE(d.phi_range > 0 && std::isfinite(d.phi_range), SFX_E_INVALID,
+                  "range_restricted molecule needs phi_range > 0");
+      const int n[3] = {lat.d.nx, lat.d.ny, lat.d.nz};
+      for (int a = 0; a < lat.d.grad; a++)
+        SFX_REQUIRE(d.range_lo[a] >= 1 && d.range_lo[a] <= d.range_hi[a] && d.range_hi[a] <= n[a], SFX_E_INVALID,
+                    "range_restricted box must lie inside the lattice (1-based, lo <= hi)");
+      for (int a = 0; a < 3; a++) {
+        mo.range.lo[a] = d.range_lo[a];
+        mo.range.hi[a] = d.range_hi[a];
+      }
+      mo.phi_range = d.phi_range;
+    }
     bool homo = std::all_of(mo.seq.begin(), mo.seq.end(), [&](int t) { return t == mo.seq[0]; });
     mo.kind = mo.N == 1 ? 0 : ((homo || is_palindrome(mo.seq)) ? 2 : 1);
     if (lat.d.order == 2 && mo.kind != 0) mo.kind = 3;  // full six-component sweep (SF_Homopol2ndO.cpp:152-207)
@@ -835,6 +850,7 @@ void System::compute_phi(int prec) {
         sweep<float, float>(m, true);
       else
         sweep<double, double>(m, false);
+      if (mols[m].freedom == SFX_RANGE_RESTRICTED) finish_range(m);
     }
   launch_phi_total(L, ns, nm, d_phi_m, d_mol_uses, d_fprof, d_frozen, d_phi, d_phitot, stream, &sp, d_molscal,
                    d_phib_state, d_cbulk);
@@ -842,6 +858,22 @@ void System::compute_phi(int prec) {
   SFX_HIP(hipGetLastError());
   phi_valid = true;
   last_precision = prec;
+}
+
+bool System::has_range() const {
+  for (const Molecule& mo : mols)
+    if (mo.freedom == SFX_RANGE_RESTRICTED) return true;
+  return false;
+}
+
+// SF_Copol1stO.cpp:1246-1280: phiPreNorm over the range, NormPhiFree with
+// phiRange / phiPreNorm, then theta and phi_b
+void System::finish_range(int m) {
+  const Molecule& mo = mols[m];
+  double* phim = d_phi_m + (size_t)m * ns * lat.d.M;
+  const int nb = launch_range_sum(lat.d, mo.range, mo.uses_mask(), ns, phim, d_partial, stream);
+  launch_range_scale(lat.d, mo.uses_mask(), ns, d_partial, nb, mo.phi_range, mo.N, d_molscal + (size_t)m * MS_COUNT,
+                     phim, stream);
 }
 
 void System::residual_internal(int prec) {

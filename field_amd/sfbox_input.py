@@ -19,10 +19,13 @@ Supported subset, mapped onto LatticeSpec / SystemSpec:
         lastlayer), chi - Y (set on either side, SF_SegmentList.cpp:145-200),
         Pf - Y (the straight-bond weight of second order, one value)
   mol : P : composition (linear blocks '(A)12(B)28'), freedom (free,
-        restricted, solvent), phibulk, theta
+        restricted, solvent, range_restricted), phibulk, theta,
+        restricted_range (a range as above, inside the lattice), phi_range
   sys : S : matrix_approach (first_order, second_order; SF_System.cpp:24-60)
-  newton : N : method (DIIS runs DIIS; any other method runs Newton-Krylov),
-        tolerance, iterationlimit, m
+  newton : N : method (DIIS runs DIIS; pseudohessian and hessian run
+        SFNewton's pseudo-Hessian Newton; any other method, or none, runs
+        Newton-Krylov, except with a range_restricted molecule, which runs
+        SFNewton), tolerance, iterationlimit, m
   var : V : step, end_value, scale (linear, exponential), type (integer,
         real), and one `class : instance` entry naming the varied
         parameter (Vary.cpp:3-230)
@@ -208,7 +211,22 @@ def _range_mask(lat: LatticeSpec, rng: str) -> np.ndarray:
 
 _SEGMENT_UNSUPPORTED = ("pinned_range", "grafted_range", "valence", "epsilon", "state1", "state2",
                         "alpha_bulk", "external_potential")
-_MOL_UNSUPPORTED = ("phibulk_range", "theta_range", "n", "n_excess", "phi_range")
+_MOL_UNSUPPORTED = ("phibulk_range", "theta_range", "n", "n_excess")
+
+
+def _range_box(lat: LatticeSpec, rng: str):
+    """(lo, hi) interior coordinates (1-based, inclusive, three entries) of a
+    range: the sites of _range_mask, which must all be interior."""
+    m = _range_mask(lat, rng).reshape(lat.dims)
+    idx = np.argwhere(m)
+    lo = [int(v) for v in idx.min(axis=0)][:lat.gradients]
+    hi = [int(v) for v in idx.max(axis=0)][:lat.gradients]
+    for k in range(lat.gradients):
+        if lo[k] < 1 or hi[k] > int(lat.n_layers[k]):
+            raise InputError(f"range '{rng}': a restricted range must lie inside the lattice")
+    lo += [1] * (3 - len(lo))
+    hi += [1] * (3 - len(hi))
+    return tuple(lo[:3]), tuple(hi[:3])
 
 
 def build(inp: Input, calc: int = 0):
@@ -289,6 +307,13 @@ def build(inp: Input, calc: int = 0):
             mols.append(MoleculeSpec(m, comp, "restricted", theta=_real(g("theta"), f"mol {m} theta")))
         elif fr == "solvent":
             mols.append(MoleculeSpec(m, comp, "solvent"))
+        elif fr == "range_restricted":
+            # SF_Molecule.cpp:169-180
+            if g("restricted_range") is None or g("phi_range") is None:
+                raise InputError(f"mol {m}: a range_restricted molecule needs restricted_range and phi_range")
+            lo, hi = _range_box(lat, g("restricted_range"))
+            mols.append(MoleculeSpec(m, comp, "range_restricted", range_lo=lo, range_hi=hi,
+                                     phi_range=_real(g("phi_range"), f"mol {m} phi_range")))
         else:
             raise InputError(f"mol {m} freedom '{fr}' is not supported")
     if not mols:
@@ -299,11 +324,14 @@ def build(inp: Input, calc: int = 0):
     nn = inp.names(calc, "newton")
     if nn:
         g = lambda p, d=None: vals.get(("newton", nn[0], p), d)
-        solver["method"] = "diis" if g("method", "").upper() == "DIIS" else "nk"
+        meth = g("method", "").lower()
+        solver["method"] = {"diis": "diis", "pseudohessian": "newton", "hessian": "newton"}.get(meth, "nk")
         solver["tolerance"] = _real(g("tolerance", "1e-7"), "newton tolerance")
         solver["max_iter"] = _int(g("iterationlimit", "1000"), "newton iterationlimit")
         if g("m") is not None and solver["method"] == "diis":
             solver["diis_m"] = _int(g("m"), "newton m")
+    if solver["method"] == "nk" and any(m.freedom == "range_restricted" for m in mols):
+        solver["method"] = "newton"  # no Jacobian products for range_restricted molecules
     return lat, sysd, solver
 
 

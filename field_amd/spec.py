@@ -83,9 +83,13 @@ def parse_composition(comp: str, states: Sequence[str]) -> List[int]:
 class MoleculeSpec:
     name: str
     composition: str
-    freedom: str = "free"          # free / restricted / solvent
+    freedom: str = "free"          # free / restricted / solvent / range_restricted
     phibulk: float = 0.0
     theta: float = 0.0
+    # range_restricted: sum of phi over the box lo..hi (1-based, inclusive) is phi_range
+    range_lo: Sequence[int] = (1, 1, 1)
+    range_hi: Sequence[int] = (1, 1, 1)
+    phi_range: float = 0.0
 
 
 @dataclass
@@ -113,13 +117,18 @@ class SystemSpec:
             prof[i] = np.asarray(p, dtype=np.float64).reshape(-1)
         seqs = [np.ascontiguousarray(np.asarray(s, dtype=np.int32)) for s in self.sequences()]
         mols = (_abi.MoleculeDesc * len(self.molecules))()
-        fmap = {"free": _abi.FREE, "restricted": _abi.RESTRICTED, "solvent": _abi.SOLVENT}
+        fmap = {"free": _abi.FREE, "restricted": _abi.RESTRICTED, "solvent": _abi.SOLVENT,
+                "range_restricted": _abi.RANGE_RESTRICTED}
         for k, (m, s) in enumerate(zip(self.molecules, seqs)):
             mols[k].length = len(s)
             mols[k].sequence = s.ctypes.data_as(C.POINTER(C.c_int))
             mols[k].freedom = fmap[m.freedom]
             mols[k].phibulk = m.phibulk
             mols[k].theta = m.theta
+            for a in range(3):
+                mols[k].range_lo[a] = int(m.range_lo[a]) if a < len(m.range_lo) else 1
+                mols[k].range_hi[a] = int(m.range_hi[a]) if a < len(m.range_hi) else 1
+            mols[k].phi_range = m.phi_range
         d = _abi.SystemDesc()
         d.n_states = ns
         d.chi = chi.ctypes.data_as(C.POINTER(C.c_double))

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define SFX_ABI_VERSION 2
+#define SFX_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -77,14 +77,24 @@ typedef struct sfx_lattice_desc {
 } sfx_lattice_desc;
 
 /* ---- system (one SCF problem on one lattice) ---------------------------- */
-enum { SFX_FREE = 0, SFX_RESTRICTED = 1, SFX_SOLVENT = 2 };
+enum { SFX_FREE = 0, SFX_RESTRICTED = 1, SFX_SOLVENT = 2, SFX_RANGE_RESTRICTED = 3 };
 
 typedef struct sfx_molecule_desc {
   int length;             /* chain length N */
   const int* sequence;    /* N state indices (0-based); rank s=1..N uses sequence[s-1] */
-  int freedom;            /* SFX_FREE (phibulk), SFX_RESTRICTED (theta), SFX_SOLVENT */
+  int freedom;            /* SFX_FREE (phibulk), SFX_RESTRICTED (theta), SFX_SOLVENT,
+                             SFX_RANGE_RESTRICTED (phi_range in range_lo..range_hi) */
   double phibulk;         /* SFX_FREE: bulk volume fraction */
   double theta;           /* SFX_RESTRICTED: amount, sum over interior sites of phi */
+  /* SFX_RANGE_RESTRICTED (freedom range_restricted, SF_Molecule.cpp:169-180;
+   * SF_Copol1stO.cpp:1246-1280): the molecule's phi, summed over its states
+   * and over the interior sites of the box range_lo..range_hi (1-based
+   * coordinates per gradient, inclusive; unused axes ignored), equals
+   * phi_range; theta and phi_b follow from it. Residuals and solves by DIIS
+   * and SFNewton; JVP and VJP return SFX_E_UNSUPPORTED. */
+  int range_lo[3];
+  int range_hi[3];
+  double phi_range;
 } sfx_molecule_desc;
 
 /* Residual / bookkeeping conventions:

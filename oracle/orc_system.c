@@ -34,6 +34,8 @@ typedef struct {
   int* seq;
   double phib, theta, lnGN, lnC, chemIntRef;
   double* phi; /* ns*M, this molecule's contribution per state */
+  int rlo[3], rhi[3]; /* range_restricted: the box, 1-based interior coordinates */
+  double phi_range;
 } omol;
 
 struct orc_sys {
@@ -119,6 +121,11 @@ orc_sys* orc_sys_new(const sfx_lattice_desc* ld, const sfx_system_desc* sd) {
     o->freedom = d->freedom;
     o->phib = d->phibulk;
     o->theta = d->theta;
+    for (int a = 0; a < 3; a++) {
+      o->rlo[a] = d->range_lo[a];
+      o->rhi[a] = d->range_hi[a];
+    }
+    o->phi_range = d->phi_range;
     o->seq = malloc(sizeof(int) * o->N);
     memcpy(o->seq, d->sequence, sizeof(int) * o->N);
     int homo = 1;
@@ -238,6 +245,42 @@ static void scale(orc_sys* s, double* phi, double c) {
   for (long z = 0; z < s->M; z++) phi[z] *= c;
 }
 
+/* rangeRestricted (SF_Copol1stO.cpp:1246-1280, SF_Homopol1stO.cpp:604-615,
+ * SF_Monomer1stO.cpp): phi_raw is the molecule's un-normalised profile per
+ * state times `raw`; its sum over the states and the interior sites of the
+ * box is brought to phi_range (NormPhiFree with phiRange / phiPreNorm), then
+ * theta = ComputeTheta (SF_Molecule.cpp:707-725: sum of site volume x phi over
+ * the interior) and phi_b = theta / G_N. */
+static int in_range(const orc_sys* s, const omol* o, long i) {
+  const orc_lat* L = s->L;
+  if (!orc_lat_is_interior(L, i)) return 0;
+  int c[3] = {(int)(i / L->jx), (int)((i / L->MZ) % L->MY), (int)(i % L->MZ)};
+  for (int a = 0; a < L->grad; a++)
+    if (c[a] < o->rlo[a] || c[a] > o->rhi[a]) return 0;
+  return 1;
+}
+
+static void range_normalize(orc_sys* s, omol* o, double raw) {
+  const long M = s->M;
+  double pre = 0;
+  for (int t = 0; t < s->ns; t++)
+    for (long z = 0; z < M; z++)
+      if (in_range(s, o, z)) pre += raw * o->phi[t * M + z];
+  const double C = o->phi_range / pre;
+  o->lnC = log(C);
+  for (int t = 0; t < s->ns; t++) scale(s, o->phi + t * M, raw * C);
+  double* tot = calloc(M, sizeof(double));
+  for (int t = 0; t < s->ns; t++)
+    for (long z = 0; z < M; z++) tot[z] += o->phi[t * M + z];
+  orc_subtract_boundaries(s->L, tot);
+  orc_multiply_sites(s->L, tot);
+  double theta = 0;
+  for (long z = 0; z < M; z++) theta += tot[z];
+  free(tot);
+  o->theta = theta;
+  o->phib = theta / exp(o->lnGN);
+}
+
 /* SF_Homopol2ndO.cpp:152-207 (full sweep over the direction components:
  * six in 3-D, the lumped five in 2-D and three in 1-D, each weighted by its
  * multiplicity in the composition) */
@@ -272,6 +315,10 @@ static void mol_compute_phi2(orc_sys* s, omol* o) {
     if (used) correct_dc(s, o->phi + t * M, s->swf + t * M);
   }
   double c;
+  if (o->freedom == SFX_RANGE_RESTRICTED) {
+    range_normalize(s, o, 1.0 / 6.0);
+    return;
+  }
   if (o->freedom == SFX_RESTRICTED) {
     o->lnC = log(o->theta) - o->lnGN - log(1.0 * N);
     o->phib = o->theta / exp(o->lnGN);
@@ -298,6 +345,10 @@ static void mol_compute_phi(orc_sys* s, omol* o) {
     G = s->swf + t * M;
     memcpy(o->phi + t * M, G, sizeof(double) * M);
     o->lnGN = orc_lngn(s->L, G);
+    if (o->freedom == SFX_RANGE_RESTRICTED) {
+      range_normalize(s, o, 1.0);
+      return;
+    }
     if (o->freedom == SFX_RESTRICTED) {
       o->lnC = log(o->theta) - o->lnGN;
       scale(s, o->phi + t * M, o->theta / exp(o->lnGN));
@@ -354,6 +405,10 @@ static void mol_compute_phi(orc_sys* s, omol* o) {
   }
   double f = half ? 2.0 : 1.0;
   double c;
+  if (o->freedom == SFX_RANGE_RESTRICTED) {
+    range_normalize(s, o, f);
+    return;
+  }
   if (o->freedom == SFX_RESTRICTED) {
     o->lnC = log(o->theta) - o->lnGN - log(1.0 * N);
     o->phib = o->theta / exp(o->lnGN);

@@ -35,10 +35,10 @@ def test_binding_covers_every_declared_symbol():
 
 def test_abi_version_and_struct_sizes():
     from field_amd import _abi, sfx
-    assert sfx.lib().sfx_abi_version() == 2
+    assert sfx.lib().sfx_abi_version() == 3
     # lattice desc: 4 ints + 3*3 ints + 4 doubles
     assert ctypes.sizeof(_abi.LatticeDesc) == 4 * 4 + 9 * 4 + 4 + 4 * 8  # 4 bytes padding before doubles
-    assert ctypes.sizeof(_abi.MoleculeDesc) == 40
+    assert ctypes.sizeof(_abi.MoleculeDesc) == 72  # + range box (6 ints) and phi_range
 
 
 def test_ctypes_layouts_match_the_c_headers(tmp_path):
