@@ -2,9 +2,12 @@
 #pragma once
 
 #include <algorithm>
+#include <array>
 #include <memory>
 #include <utility>
+#include <vector>
 
+#include "sfx_chain1d.h"
 #include "sfx_kernels.h"
 
 namespace sfx {
@@ -144,6 +147,14 @@ struct System {
   int store_slots = 1;     // rank slots of the propagator store (all molecules share it)
   bool checkpointed = false;
   bool use_block3 = false;  // blocked 3-D simple-cubic sweeps (sfx_options.sweep_kernel)
+  bool use_chain1 = false;  // 1-D FP64 chains in one launch per stretch (sfx_options.sweep_kernel)
+  // per molecule and stretch (0 forward, 1 backward, 2 half-store second half):
+  // the step list on the device, built on the first sweep (its pointers never change)
+  std::vector<std::array<Chain1Step*, 3>> chain1_dev;
+  std::vector<std::array<int, 3>> chain1_n;
+  // one launch of the stretch `which` of molecule m (building its list on
+  // first use from the host steps)
+  void chain1_run(int m, int which, const std::vector<Chain1Step>& host, const double* in0, double* last);
   SysParams sp{};
   hipStream_t stream = nullptr;      // where every call of this system runs
   hipStream_t own_stream = nullptr;  // created with the system, used unless a caller sets one

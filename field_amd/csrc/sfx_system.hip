@@ -63,6 +63,9 @@ static T* dalloc(size_t n) {
 
 System::~System() {
   drop_graphs();
+  for (auto& a : chain1_dev)
+    for (Chain1Step* p : a)
+      if (p) (void)hipFree(p);
   void* ptrs[] = {d_u, d_f, d_swf, d_swf32, d_phi_m, d_acc, d_acc2, d_phi, d_rho, d_phitot, d_fprof, d_frozen,
                   d_st_itvar, d_mol_uses, d_itmask, d_sites, d_store, d_pp, d_pp2, d_molscal, d_lnf, d_lnb,
                   d_maxf, d_maxb, d_partial, d_phib_state, d_cbulk, d_scalar, d_xabi, d_gmax,
@@ -138,6 +141,7 @@ void System::create(const sfx_lattice_desc& ld, const sfx_system_desc& sd, const
     const int sk = opt ? opt->sweep_kernel : SFX_SWEEP_AUTO;
     SFX_REQUIRE(sk >= SFX_SWEEP_AUTO && sk <= SFX_SWEEP_BLOCKED, SFX_E_INVALID, "unknown sweep_kernel");
     use_block3 = sk != SFX_SWEEP_PER_RANK && block3_ok(lat.d);
+    use_chain1 = sk != SFX_SWEEP_PER_RANK && chain1d_ok(lat.d);
     schedule = opt ? opt->chain_schedule : SFX_SCHEDULE_AUTO;
     SFX_REQUIRE(schedule == SFX_SCHEDULE_AUTO || schedule == SFX_SCHEDULE_SEQUENTIAL, SFX_E_INVALID,
                 "unknown chain_schedule");
@@ -529,20 +533,46 @@ void System::sweep(int m, bool scaled) {
     launch_prop<T, A>(L, slot(src), gp(mo.seq[s - 1]), slot(dst), nullptr, nullptr, fsc(s, 0), st);
     prof.end(st);
   };
-  chain_forward(plan, step);  // ranks 2..N (full sweep) or 2..N/2 (half store)
+  // 1-D, FP64, full store: every stretch of ranks in one workgroup launch
+  // (sfx_chain1d.hip); kernel timing, which reports per-rank launches, keeps
+  // the per-rank path
+  bool chain1 = false;
+  if constexpr (std::is_same<T, double>::value && std::is_same<A, double>::value)
+    chain1 = use_chain1 && !scaled && !plan.c && !prof.on;
+  if (chain1) {
+    if constexpr (std::is_same<T, double>::value && std::is_same<A, double>::value) {
+      std::vector<Chain1Step> hs;
+      chain_forward(plan, [&](int s, int src, int dst) {
+        SFX_REQUIRE(src == s - 2 && dst == s - 1, SFX_E_INVALID, "full-store chain expected");
+        hs.push_back(Chain1Step{gp(mo.seq[s - 1]), slot(dst), nullptr, nullptr});
+      });
+      chain1_run(m, 0, hs, slot(0), nullptr);
+    }
+  } else {
+    chain_forward(plan, step);  // ranks 2..N (full sweep) or 2..N/2 (half store)
+  }
   ChainReader<decltype(step)> rd{plan, step};
   if (mo.kind == 1) {
     k_set_ref<<<1, 1, 0, st>>>(scal, scaled ? lnf : nullptr, N, 1.0);
     // backward: step k = N-s+1 holds G_inv(s)
     launch_init_rank<T>(L, gp(mo.seq[N - 1]), pp[0], bsc(1, N), st);
     launch_connect<T, A>(L, pp[0], slot(rd.get(N)), accp(mo.seq[N - 1]), bsc(1, N), 1.0, st);
-    for (int s = N - 1; s >= 1; s--) {
-      int k = N - s + 1;
-      const T* pair = slot(rd.get(s));
-      prof.begin(1, P, st);
-      launch_prop<T, A>(L, pp[(k - 2) & 1], gp(mo.seq[s - 1]), pp[(k - 1) & 1], pair, accp(mo.seq[s - 1]),
-                        bsc(k, s), st);
-      prof.end(st);
+    if (chain1) {
+      if constexpr (std::is_same<T, double>::value && std::is_same<A, double>::value) {
+        std::vector<Chain1Step> hs;
+        for (int s = N - 1; s >= 1; s--)
+          hs.push_back(Chain1Step{gp(mo.seq[s - 1]), nullptr, slot(rd.get(s)), accp(mo.seq[s - 1])});
+        chain1_run(m, 1, hs, pp[0], pp[(N - 1) & 1]);
+      }
+    } else {
+      for (int s = N - 1; s >= 1; s--) {
+        int k = N - s + 1;
+        const T* pair = slot(rd.get(s));
+        prof.begin(1, P, st);
+        launch_prop<T, A>(L, pp[(k - 2) & 1], gp(mo.seq[s - 1]), pp[(k - 1) & 1], pair, accp(mo.seq[s - 1]),
+                          bsc(k, s), st);
+        prof.end(st);
+      }
     }
     final_G = pp[(N - 1) & 1];
     lnfinal = scaled ? lnb + N : nullptr;
@@ -559,13 +589,24 @@ void System::sweep(int m, bool scaled) {
       RankScale c = fsc(s, s);
       launch_connect<T, A>(L, pp[cur], pp[cur], accp(mo.seq[s - 1]), c, 0.5, st);
     }
-    for (int s = (N + 3) / 2; s <= N; s++) {
-      int sinv = N - s + 1;
-      const T* pair = slot(rd.get(sinv));
-      prof.begin(1, P, st);
-      launch_prop<T, A>(L, pp[cur], gp(mo.seq[s - 1]), pp[cur ^ 1], pair, accp(mo.seq[s - 1]), fsc(s, sinv), st);
-      prof.end(st);
-      cur ^= 1;
+    if (chain1) {
+      if constexpr (std::is_same<T, double>::value && std::is_same<A, double>::value) {
+        std::vector<Chain1Step> hs;
+        for (int s = (N + 3) / 2; s <= N; s++)
+          hs.push_back(Chain1Step{gp(mo.seq[s - 1]), nullptr, slot(rd.get(N - s + 1)), accp(mo.seq[s - 1])});
+        const int fin = cur ^ (int)(hs.size() & 1);
+        chain1_run(m, 2, hs, pp[cur], pp[fin]);
+        cur = fin;
+      }
+    } else {
+      for (int s = (N + 3) / 2; s <= N; s++) {
+        int sinv = N - s + 1;
+        const T* pair = slot(rd.get(sinv));
+        prof.begin(1, P, st);
+        launch_prop<T, A>(L, pp[cur], gp(mo.seq[s - 1]), pp[cur ^ 1], pair, accp(mo.seq[s - 1]), fsc(s, sinv), st);
+        prof.end(st);
+        cur ^= 1;
+      }
     }
     final_G = pp[cur];
     lnfinal = scaled ? lnf + N : nullptr;
@@ -858,6 +899,25 @@ void System::compute_phi(int prec) {
   SFX_HIP(hipGetLastError());
   phi_valid = true;
   last_precision = prec;
+}
+
+void System::chain1_run(int m, int which, const std::vector<Chain1Step>& host, const double* in0, double* last) {
+  if (host.empty()) {
+    if (last && last != in0) launch_copy_interior<double>(lat.d, in0, last, stream);
+    return;
+  }
+  if (chain1_dev.size() < (size_t)nm) {
+    chain1_dev.resize(nm, {nullptr, nullptr, nullptr});
+    chain1_n.resize(nm, {0, 0, 0});
+  }
+  Chain1Step*& d = chain1_dev[m][which];
+  if (!d) {
+    d = dalloc<Chain1Step>(host.size());
+    SFX_HIP(hipMemcpy(d, host.data(), host.size() * sizeof(Chain1Step), hipMemcpyHostToDevice));
+    chain1_n[m][which] = (int)host.size();
+  }
+  SFX_REQUIRE(chain1_n[m][which] == (int)host.size(), SFX_E_INVALID, "chain step list changed");
+  launch_chain1d(lat.d, in0, d, (int)host.size(), last, stream);
 }
 
 bool System::has_range() const {

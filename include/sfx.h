@@ -130,7 +130,10 @@ typedef struct sfx_options {
   int sweep_kernel;       /* first-order 3-D simple-cubic sweeps: SFX_SWEEP_AUTO,
                              SFX_SWEEP_PER_RANK (one launch per rank) or
                              SFX_SWEEP_BLOCKED (up to four ranks per launch through
-                             LDS, where every side exceeds four layers) */
+                             LDS, where every side exceeds four layers); on 1-D
+                             first-order lattices of at most 4000 layers, AUTO and
+                             BLOCKED run each FP64 stretch of a chain's ranks in
+                             one workgroup launch (same bits as per rank) */
   int chain_schedule;     /* SFX_SCHEDULE_AUTO: a first-order chain without mirror
                              symmetry, fully stored, on a blocked lattice advances its
                              forward and backward chains in the same launches (each
