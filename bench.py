@@ -222,6 +222,8 @@ def main():
     ap.add_argument("--batch-max-iter", type=int, default=5000)
     ap.add_argument("--transfer-compare", action="store_true",
                     help="only time the second-order transfer as rows and on the matrix cores (128^3), then exit")
+    ap.add_argument("--wall-only", action="store_true",
+                    help="only solve the 100-layer wall adsorption problem (SFNewton and DIIS), then exit")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one untimed FP32 elementwise copy of this many MiB first (a known-traffic "
                          "kernel that calibrates hardware counters read in the same run)")
@@ -233,6 +235,9 @@ def main():
 
     if args.transfer_compare:
         transfer_compare(args)
+        return
+    if args.wall_only:
+        print(json.dumps({"wall_adsorption": wall_adsorption_leg(args)}))
         return
 
     import numpy as np
