@@ -276,7 +276,7 @@ __device__ bool batch_diis(const BatchProb& p, int b, double* __restrict__ x, co
   __shared__ double red[4 * (BT / 64)];
   __shared__ double dts[kMaxDiis + 1];
   __shared__ double Ap[kMaxDiis * kMaxDiis], rhs[kMaxDiis], coef[kMaxDiis];
-  __shared__ int verdict, save_best, finished;
+  __shared__ int decision, save_best, finished;
   const long long nv = 2LL * p.n;
   double* xb = x + p.xoff;
   const double* gb = g + p.xoff;
@@ -386,12 +386,12 @@ __device__ bool batch_diis(const BatchProb& p, int b, double* __restrict__ x, co
       if (it >= max_iter) ns.done = 2;
     }
     S[b] = ns;
-    verdict = v;
+    decision = v;
     save_best = save;
     finished = ns.done != 0;
   }
   __syncthreads();
-  const int vd = verdict;
+  const int vd = decision;
   const bool done = finished != 0;
   if (save_best)
     for (long long q = t; q < nv; q += BT) xbest[q] = x0[q];  // x0 holds the evaluated point
@@ -410,7 +410,7 @@ __device__ bool batch_diis(const BatchProb& p, int b, double* __restrict__ x, co
       xb[q] = s;
     }
   }  // vd == 1: x stays the point whose residual was measured
-  // the shared verdict words are rewritten by the next step's thread 0 only
+  // the shared decision words are rewritten by the next step's thread 0 only
   // after every thread has read them here
   __syncthreads();
   return done;

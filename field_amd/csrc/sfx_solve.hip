@@ -309,7 +309,7 @@ void solve_diis(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
   (void)hipEventDestroy(e1);
   // the residual of the returned x (leaving phi / free energy consistent
   // with it): at the iteration limit x is the last, untested DIIS
-  // combination, so the norm and the verdict come from this evaluation
+  // combination, so the norm and the convergence flag come from this evaluation
   S.residual_replay(SFX_FP64);
   const double fin = std::sqrt(V.dot(g, g));
   st.iterations = it;
