@@ -224,8 +224,6 @@ def main():
                     help="only time the second-order transfer as rows and on the matrix cores (128^3), then exit")
     ap.add_argument("--wall-only", action="store_true",
                     help="only solve the 100-layer wall adsorption problem (SFNewton and DIIS), then exit")
-    ap.add_argument("--event-stride", type=int, default=4,
-                    help="in the timed sweeps, bracket every N-th launch of the dominant kernel with HIP events")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one untimed FP32 elementwise copy of this many MiB first (a known-traffic "
                          "kernel that calibrates hardware counters read in the same run)")
@@ -266,15 +264,14 @@ def main():
     torch.cuda.synchronize()
     # one untimed sweep with every propagator launch bracketed by HIP events
     # gives the per-kernel table and names the dominant kernel; the timed
-    # sweeps bracket only every event_stride-th of that kernel's launches
-    # (each event pair costs queue time the solve would not otherwise spend)
+    # sweeps bracket only that kernel's launches (each event pair costs queue
+    # time the solve would not otherwise spend)
     G.profile_start()
     sweep(G, xdev, path, opts, cont)
     torch.cuda.synchronize()
     prof_all = G.profile_stop()
     dom_all = dominant(prof_all)
-    G.profile_start(only=[(dom_all[0]["kind"], dom_all[0]["precision"])] if dom_all else [],
-                    stride=args.event_stride)
+    G.profile_start(only=[(dom_all[0]["kind"], dom_all[0]["precision"])] if dom_all else [])
     stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -345,7 +342,7 @@ def main():
                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(d, args.n),
                     "kernel": f"{d['kernel']} ({d['precision']})", "avg_us": round(d["avg_us"], 3),
                     "launches": d["launches"], "ranks_per_launch": round(d["ranks"] / d["launches"], 3),
-                    "bytes_per_launch": d["bytes_per_launch"], "timed_every": args.event_stride}
+                    "bytes_per_launch": d["bytes_per_launch"]}
 
     batch = None if args.no_batch else batch_leg(args)
     wall = None if args.no_batch else wall_adsorption_leg(args)

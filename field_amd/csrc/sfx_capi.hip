@@ -283,18 +283,13 @@ int sfx_newton_iterate(sfx_residual_fn f, void* ctx, double* x, long n, const sf
 // ---- kernel timing (include/sfx_profile.h) ----------------------------------
 int sfx_profile_start(sfx_system* s) { return sfx_profile_start_select(s, ~0u); }
 
-int sfx_profile_start_select(sfx_system* s, unsigned select) { return sfx_profile_start_sampled(s, select, 1u); }
-
-int sfx_profile_start_sampled(sfx_system* s, unsigned select, unsigned stride) {
+int sfx_profile_start_select(sfx_system* s, unsigned select) {
   return guarded([&] {
     SFX_REQUIRE(s, SFX_E_INVALID, "NULL system");
-    SFX_REQUIRE(stride >= 1, SFX_E_INVALID, "profile stride must be at least 1");
     Profiler& p = s->s.prof;
     p.marks.clear();
     p.used = 0;
     p.select = select;
-    p.stride = stride;
-    p.seen = 0;
     p.open = false;
     p.on = true;
   });

@@ -116,9 +116,6 @@ struct Profiler {
   bool times(int k0, int k1) const {
     return wants(k0, SFX_FP64) || wants(k1, SFX_FP64) || wants(k0, SFX_FP32) || wants(k1, SFX_FP32);
   }
-  unsigned stride = 1;  // time the first selected launch and every stride-th after it
-  unsigned long long seen = 0;  // selected launches so far
-  bool sample() { return seen++ % stride == 0; }
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   struct Mark {

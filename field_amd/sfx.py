@@ -46,7 +46,6 @@ _SIGS = {
     "sfx_precondition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "sfx_profile_start": (C.c_int, [C.c_void_p]),
     "sfx_profile_start_select": (C.c_int, [C.c_void_p, C.c_uint]),
-    "sfx_profile_start_sampled": (C.c_int, [C.c_void_p, C.c_uint, C.c_uint]),
     "sfx_profile_stop": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sfx_batch_create": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
     "sfx_batch_destroy": (C.c_int, [C.c_void_p]),
@@ -298,17 +297,16 @@ class System:
         self._stream = stream
         check(lib().sfx_stream_set(self.h, C.c_void_p(ptr)))
 
-    def profile_start(self, only=None, stride=1):
+    def profile_start(self, only=None):
         """Time propagator launches with HIP events until profile_stop; `only`:
-        a list of (kernel kind, "fp32"/"fp64") pairs to time, the rest untimed;
-        `stride`: time every stride-th of the selected launches."""
-        if only is None and stride == 1:
+        a list of (kernel kind, "fp32"/"fp64") pairs to time, the rest untimed."""
+        if only is None:
             check(lib().sfx_profile_start(self.h))
             return
-        sel = ~0 & 0xFFFFFFFF if only is None else 0
-        for kind, prec in only or []:
+        sel = 0
+        for kind, prec in only:
             sel |= 1 << (2 * kind + (1 if prec == "fp32" else 0))
-        check(lib().sfx_profile_start_sampled(self.h, C.c_uint(sel), C.c_uint(stride)))
+        check(lib().sfx_profile_start_select(self.h, C.c_uint(sel)))
 
     def profile_stop(self):
         """Per kernel kind and precision: launches, summed HIP-event ms, bytes per launch."""

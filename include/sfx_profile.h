@@ -49,10 +49,6 @@ int sfx_profile_start(sfx_system* s);
  * 2*kind + precision is set in `select`; the other launches run without
  * events (every event pair adds a few microseconds of queue time). */
 int sfx_profile_start_select(sfx_system* s, unsigned select);
-/* As sfx_profile_start_select, timing only every `stride`-th launch of the
- * selected pairs (the first, then every stride-th after it); stride 1 times
- * them all. The report's launches count the timed launches. */
-int sfx_profile_start_sampled(sfx_system* s, unsigned select, unsigned stride);
 int sfx_profile_stop(sfx_system* s, sfx_profile_report* report);
 
 #ifdef __cplusplus

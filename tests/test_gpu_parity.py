@@ -309,16 +309,6 @@ def test_profile_select_times_only_the_chosen_kernel(sfx):
     assert sel[0]["total_ms"] > 0
     for a, b in ((f0, f1), (f0, f2), (j0, j1), (j0, j2)):
         assert np.array_equal(a, b)
-    # sfx_profile_start_sampled: the first selected launch and every third after it
-    G.profile_start(only=[(3, "fp32")], stride=3)
-    f3, j3 = G.residual(x0), G.jvp(x0, v)
-    smp = G.profile_stop()
-    n = full[(3, "fp32")]["launches"]
-    assert n >= 2 and [(e["kind"], e["precision"]) for e in smp] == [(3, "fp32")]
-    assert smp[0]["launches"] == (n + 2) // 3 and smp[0]["total_ms"] > 0
-    assert np.array_equal(f0, f3) and np.array_equal(j0, j3)
-    with pytest.raises(sfx.SFError):
-        G.profile_start(only=[(3, "fp32")], stride=0)
 
 
 def test_chi_sweep_reaches_melt_solution(sfx, orc):
