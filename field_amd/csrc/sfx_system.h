@@ -111,11 +111,6 @@ struct Profiler {
   unsigned select = ~0u;  // bit 2*kind + precision: that pair is timed
   bool open = false;      // begin() recorded a start event that end() closes
   bool wants(int kind, int prec) const { return on && (select >> (2 * kind + prec) & 1u); }
-  // kinds k0 or k1 timed in either precision: launches of those kinds run
-  // plainly (events cannot be recorded into a captured graph)
-  bool times(int k0, int k1) const {
-    return wants(k0, SFX_FP64) || wants(k1, SFX_FP64) || wants(k0, SFX_FP32) || wants(k1, SFX_FP32);
-  }
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   struct Mark {
@@ -224,7 +219,7 @@ struct System {
   bool has_range() const;
   // residual_internal replayed from a HIP graph captured on the first call
   // (per precision): one submission instead of a launch per rank. Plain
-  // launches while the propagate / compose kinds are timed. Kernels take their parameters by
+  // launches while kernel timing is on. Kernels take their parameters by
   // value, so set_chi drops the graphs.
   void residual_replay(int precision);
   void drop_graphs();

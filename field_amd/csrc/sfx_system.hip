@@ -538,7 +538,7 @@ void System::sweep(int m, bool scaled) {
   // the per-rank path
   bool chain1 = false;
   if constexpr (std::is_same<T, double>::value && std::is_same<A, double>::value)
-    chain1 = use_chain1 && !scaled && !plan.c && !prof.times(0, 1);
+    chain1 = use_chain1 && !scaled && !plan.c && !prof.on;
   if (chain1) {
     if constexpr (std::is_same<T, double>::value && std::is_same<A, double>::value) {
       std::vector<Chain1Step> hs;
@@ -952,7 +952,7 @@ void System::drop_graphs() {
 
 void System::residual_replay(int prec) {
   const int k = prec == SFX_FP32 ? 1 : 0;
-  if (prof.times(0, 1) || !graph_ok) {
+  if (prof.on || !graph_ok) {
     residual_internal(prec);
     return;
   }

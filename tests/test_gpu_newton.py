@@ -88,24 +88,3 @@ def test_newton_refuses_large_systems(sfx):
         G.solve(x0, method="newton")
     assert e.value.code == _abi.SFX_E_UNSUPPORTED
 
-
-def test_graphs_replay_while_other_kinds_are_timed(sfx):
-    """Timing only the tangent compose launches (the bench's timed sweeps)
-    leaves the residual graph in use: the solve replays it and gives the bits
-    of the untimed solve; timing the compose kind of the residual launches
-    every kernel instead."""
-    lat, sysd, x0 = configs.adsorption_1d()
-    G = sfx.System(lat, sysd)
-    xg, sg = G.solve(x0, method="newton", tolerance=1e-9, max_iter=1000)
-    H = sfx.System(lat, sysd)
-    H.profile_start(only=[(3, "fp32")])
-    xt, stt = H.solve(x0, method="newton", tolerance=1e-9, max_iter=1000)
-    H.profile_stop()
-    assert stt["graph_replays"] == stt["residual_evals"] - 1
-    assert np.array_equal(xt, xg)
-    K = sfx.System(lat, sysd)
-    K.profile_start(only=[(1, "fp64")])
-    xk, sk = K.solve(x0, method="newton", tolerance=1e-9, max_iter=1000)
-    K.profile_stop()
-    assert sk["graph_replays"] == 0
-    assert np.array_equal(xk, xg)
