@@ -87,4 +87,3 @@ def test_newton_refuses_large_systems(sfx):
     with pytest.raises(sfx.SFError) as e:
         G.solve(x0, method="newton")
     assert e.value.code == _abi.SFX_E_UNSUPPORTED
-
