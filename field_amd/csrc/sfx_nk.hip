@@ -134,7 +134,7 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
               dl.a[i] = Vk(base + i);
               dl.b[i] = wv;
             }
-            V.dots_dev(dl, gs + pass * kGsPass + base);
+            V.dots_dev(dl, gs + pass * kGsPass + base, pass * kGsPass + base);
           }
           VecList vl{};
           vl.count = j + 2;
@@ -146,9 +146,9 @@ void solve_nk(System& S, const sfx_solver_opts& o, sfx_solve_stats& st) {
           DotList dl{};
           dl.count = 1;
           dl.a[0] = dl.b[0] = wv;
-          V.dots_dev(dl, gs + 2 * kGsPass);
+          V.dots_dev(dl, gs + 2 * kGsPass, 2 * kGsPass);
         }
-        V.fetch(gs, 2 * kGsPass + 1, hb.data());
+        V.wait_host(2 * kGsPass + 1, hb.data());
         for (int i = 0; i <= j; i++) h[i] = (0.0 + hb[i]) + hb[kGsPass + i];
         double hn = std::sqrt(hb[2 * kGsPass]);
         for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = h[i];

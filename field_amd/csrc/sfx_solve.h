@@ -50,7 +50,9 @@ struct VecOps {
   void init(long long n, hipStream_t st);  // allocates only when n changes
   ~VecOps();
   void dots(const DotList& dl, double* host_out);  // waits for the results
-  void dots_dev(const DotList& dl, double* dout);  // results stay on the device
+  // results stay on the device; with mirror_at >= 0 also in hres[mirror_at..]
+  void dots_dev(const DotList& dl, double* dout, int mirror_at = -1);
+  void wait_host(int count, double* host_out);  // synchronise, then copy hres[0..count) out
   void fetch(const double* dsrc, int count, double* host_out);  // one copy, then wait
   double dot(const double* a, const double* b);
   void combine(const VecList& vl, double* y);  // y = sum_k vl.c[k] vl.v[k]

@@ -41,8 +41,11 @@ static __global__ void __launch_bounds__(kDotBlock) k_multidot(DotList dl, long 
 }
 
 // out[r] = sum_c partial[r][c]: one workgroup per row, coalesced strided
-// loads, then a fixed-order wave and workgroup reduction
-static __global__ void __launch_bounds__(kDotBlock) k_reduce_rows(const double* partial, int ncols, double* out) {
+// loads, then a fixed-order wave and workgroup reduction; with a mirror
+// (coherent pinned host memory) the sum is stored there too, so the host
+// reads it after a stream synchronisation without a copy
+static __global__ void __launch_bounds__(kDotBlock) k_reduce_rows(const double* partial, int ncols, double* out,
+                                                                  double* mirror) {
   __shared__ double sm[kDotBlock / 64];
   const int r = blockIdx.x;
   double s = 0;
@@ -54,6 +57,7 @@ static __global__ void __launch_bounds__(kDotBlock) k_reduce_rows(const double* 
     double t = 0;
     for (int k = 0; k < kDotBlock / 64; k++) t += sm[k];
     out[r] = t;
+    if (mirror) mirror[r] = t;
   }
 }
 
@@ -112,18 +116,24 @@ void VecOps::init(long long n_, hipStream_t st) {
   nblk = (int)std::min<long long>(grid1(n, kDotBlock), 1024);
   SFX_HIP(hipMalloc(&partial, sizeof(double) * kMaxDots * nblk));
   SFX_HIP(hipMalloc(&dres, sizeof(double) * kMaxDots));
-  if (!hres) SFX_HIP(hipHostMalloc(&hres, sizeof(double) * kHostWords, hipHostMallocDefault));
+  if (!hres) SFX_HIP(hipHostMalloc(&hres, sizeof(double) * kHostWords, hipHostMallocCoherent));
 }
 VecOps::~VecOps() {
   if (partial) (void)hipFree(partial);
   if (dres) (void)hipFree(dres);
   if (hres) (void)hipHostFree(hres);
 }
-void VecOps::dots_dev(const DotList& dl, double* dout) {
+void VecOps::dots_dev(const DotList& dl, double* dout, int mirror_at) {
   SFX_REQUIRE(dl.count <= kMaxDots, SFX_E_INVALID, "too many dot products");
+  SFX_REQUIRE(mirror_at < 0 || mirror_at + dl.count <= kHostWords, SFX_E_INVALID, "host mirror out of range");
   if (dl.count <= 0) return;
   k_multidot<<<nblk, kDotBlock, 0, stream>>>(dl, n, partial);
-  k_reduce_rows<<<dl.count, kDotBlock, 0, stream>>>(partial, nblk, dout);
+  k_reduce_rows<<<dl.count, kDotBlock, 0, stream>>>(partial, nblk, dout, mirror_at < 0 ? nullptr : hres + mirror_at);
+}
+void VecOps::wait_host(int count, double* host_out) {
+  SFX_REQUIRE(count >= 0 && count <= kHostWords, SFX_E_INVALID, "too many words to read back");
+  SFX_HIP(hipStreamSynchronize(stream));
+  std::memcpy(host_out, hres, sizeof(double) * count);
 }
 void VecOps::fetch(const double* dsrc, int count, double* host_out) {
   SFX_REQUIRE(count >= 0 && count <= kHostWords, SFX_E_INVALID, "too many words to read back");
@@ -134,8 +144,8 @@ void VecOps::fetch(const double* dsrc, int count, double* host_out) {
 }
 void VecOps::dots(const DotList& dl, double* host_out) {
   if (dl.count <= 0) return;
-  dots_dev(dl, dres);
-  fetch(dres, dl.count, host_out);
+  dots_dev(dl, dres, 0);
+  wait_host(dl.count, host_out);
 }
 double VecOps::dot(const double* a, const double* b) {
   DotList dl{};
