@@ -587,8 +587,8 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
     const int tN = mo.seq[N - 1];
     tinit2<float>(L, TInit<float>{gp(mo.seq[0]), vp(mo.seq[0]), slot(0), dslot(0), fsc(1, 0)},
                   TInit<float>{gp(tN), vp(tN), slot(N - 1), dslot(N - 1), bsc(1, N)}, st);
-    const auto F1 = chain_launches(2, h, 2, state_of_rank);
-    const auto B1 = chain_launches(2, N - h, 2, bstate);
+    const auto F1 = chain_launches(2, h, 2, state_of_rank, kB3TanRanks);
+    const auto B1 = chain_launches(2, N - h, 2, bstate, kB3TanRanks);
     for (size_t i = 0; i < std::max(F1.size(), B1.size()); i++) {
       B3Launch<float> la{};
       if (i < F1.size()) {
@@ -617,8 +617,8 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
       go(la, false);
     }
     launch_set_ref_pair(scal, lnf, h, lnb, N - h, st);
-    const auto F2 = chain_launches(h + 1, N, h + 1, state_of_rank);
-    const auto B2 = chain_launches(N - h + 1, N, N - h + 1, bstate);
+    const auto F2 = chain_launches(h + 1, N, h + 1, state_of_rank, kB3TanRanks);
+    const auto B2 = chain_launches(N - h + 1, N, N - h + 1, bstate, kB3TanRanks);
     int fc = 0, bc = 0;
     for (size_t i = 0; i < std::max(F2.size(), B2.size()); i++) {
       B3Launch<float> la{};
@@ -683,7 +683,7 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
       launch(sa, sb, prev, false, mo.seq[sa - 1], maxf, lnf, outs, douts, nullptr, nullptr, false);
       if (plan.c && last == plan.seg(parity)) parity ^= 1;
       prev = last;
-    });
+    }, kB3TanRanks);
   }
   int resident[2] = {-1, -1};
   auto get = [&](int s) -> int {
@@ -702,7 +702,7 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
         }
         launch(sa, sb, prev, false, mo.seq[sa - 1], maxf, lnf, outs, douts, nullptr, nullptr, false);
         prev = plan.seg_in(set, sb - start - 1);
-      });
+      }, kB3TanRanks);
       resident[set] = j;
     }
     return plan.seg_in(set, s - start - 1);
@@ -733,7 +733,7 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
           douts[kb - ka] = dpp[cur ^ 1];
           launch(ka, kb, cur, true, mo.seq[rank_of(ka) - 1], maxb, lnb, outs, douts, pslots, pidx, true);
           cur ^= 1;
-        });
+        }, kB3TanRanks);
   } else {
     const int h = N / 2;
     k_set_ref2<<<1, 1, 0, st>>>(scal, lnf, h, 2.0);
@@ -762,7 +762,7 @@ static void tsweep_blocked(System& S, Tangent& W, int m, const float* vt, const 
           douts[sb - sa] = dpp[cur ^ 1];
           launch(sa, sb, cur, true, mo.seq[sa - 1], maxf, lnf, outs, douts, pslots, pidx, true);
           cur ^= 1;
-        });
+        }, kB3TanRanks);
   }
   fin = pp[cur];
   dfin = dpp[cur];

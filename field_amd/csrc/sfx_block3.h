@@ -9,7 +9,8 @@
 
 namespace sfx {
 
-constexpr int kB3Ranks = 4;  // ranks per launch at most
+constexpr int kB3Ranks = 4;     // ranks per launch at most
+constexpr int kB3TanRanks = 2;  // ranks per tangent launch at most (its region fits two workgroups per CU)
 
 // Ranks s0+1 .. s0+nr of one chain, all of one segment state.
 template <typename T>
@@ -46,16 +47,16 @@ struct B3Launch {
 };
 
 // Splits chain steps a..b (ascending) into launches: a launch ends at a grid
-// line ((step - g0 + 1) % kB3Ranks == 0), before a step of another segment
+// line ((step - g0 + 1) % grid == 0; grid is kB3Ranks, kB3TanRanks for tangents), before a step of another segment
 // state, or before a step that split() marks. f(first, last) per launch. The
 // grid makes the partition of any stretch that starts on a grid line equal
 // to that of the whole chain, so a recomputed stretch reproduces it bit for bit.
 template <class State, class Split, class F>
-void chain_blocks(int a, int b, int g0, State state, Split split, F f) {
+void chain_blocks(int a, int b, int g0, State state, Split split, F f, int grid = kB3Ranks) {
   int s = a;
   while (s <= b) {
     int e = s;
-    while (e < b && (e + 1 - g0) % kB3Ranks != 0 && state(e + 1) == state(s) && !split(e + 1)) e++;
+    while (e < b && (e + 1 - g0) % grid != 0 && state(e + 1) == state(s) && !split(e + 1)) e++;
     f(s, e);
     s = e + 1;
   }
@@ -63,9 +64,10 @@ void chain_blocks(int a, int b, int g0, State state, Split split, F f) {
 
 // the launches of chain_blocks(a, b, g0, state, no extra splits) as a list
 template <class State>
-std::vector<std::pair<int, int>> chain_launches(int a, int b, int g0, State state) {
+std::vector<std::pair<int, int>> chain_launches(int a, int b, int g0, State state, int grid = kB3Ranks) {
   std::vector<std::pair<int, int>> out;
-  chain_blocks(a, b, g0, state, [](int) { return false; }, [&](int s, int e) { out.emplace_back(s, e); });
+  chain_blocks(
+      a, b, g0, state, [](int) { return false; }, [&](int s, int e) { out.emplace_back(s, e); }, grid);
   return out;
 }
 
