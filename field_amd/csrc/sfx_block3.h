@@ -10,7 +10,12 @@
 namespace sfx {
 
 constexpr int kB3Ranks = 4;     // ranks per launch at most
-constexpr int kB3TanRanks = 2;  // ranks per tangent launch at most (its region fits two workgroups per CU)
+constexpr int kB3TanRanks = 2;  // ranks per tangent or FP64 launch at most (that region fits two workgroups per CU)
+// ranks per launch at most of a variant
+template <typename T>
+constexpr int b3_max_ranks(bool tangent) {
+  return (tangent || sizeof(T) == 8) ? kB3TanRanks : kB3Ranks;
+}
 
 // Ranks s0+1 .. s0+nr of one chain, all of one segment state.
 template <typename T>
@@ -47,7 +52,7 @@ struct B3Launch {
 };
 
 // Splits chain steps a..b (ascending) into launches: a launch ends at a grid
-// line ((step - g0 + 1) % grid == 0; grid is kB3Ranks, kB3TanRanks for tangents), before a step of another segment
+// line ((step - g0 + 1) % grid == 0; grid is b3_max_ranks of the variant), before a step of another segment
 // state, or before a step that split() marks. f(first, last) per launch. The
 // grid makes the partition of any stretch that starts on a grid line equal
 // to that of the whole chain, so a recomputed stretch reproduces it bit for bit.

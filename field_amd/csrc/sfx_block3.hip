@@ -34,9 +34,9 @@ namespace sfx {
 constexpr int kTX = 8, kTY = 8, kTZ = 16;
 constexpr int kH = kB3Ranks;  // largest widening: the lattice-side bound of block3_ok
 // the tile widened by h (the most ranks a launch of the variant advances)
-template <bool TANGENT>
+template <typename T, bool TANGENT>
 struct B3Region {
-  static constexpr int h = TANGENT ? kB3TanRanks : kB3Ranks;
+  static constexpr int h = b3_max_ranks<T>(TANGENT);
   static constexpr int rx = kTX + 2 * h, ry = kTY + 2 * h, rz = kTZ + 2 * h;
   static constexpr int ryz = ry * rz;
   static constexpr int r = rx * ryz;  // region sites
@@ -72,11 +72,12 @@ __device__ __forceinline__ T block_max_T(T m, float* red) {
 
 template <typename T, bool TANGENT, bool SCALED, bool CONNECT>
 // FP32 tangent variants fit two workgroups per CU in LDS; the bound asks the
-// compiler for the register budget (64 VGPRs) that lets them
+// compiler for the register budget (64 VGPRs) that lets them (the FP64 base
+// variants, also two per CU, stay below it unasked)
 __global__ void __launch_bounds__(kB3) __attribute__((amdgpu_waves_per_eu((TANGENT && sizeof(T) == 4) ? 8 : 1)))
 k_block3(DevLattice L, B3Launch<T> la, int nb) {
-  constexpr int kH = B3Region<TANGENT>::h;
-  constexpr int kRZ = B3Region<TANGENT>::rz, kRYZ = B3Region<TANGENT>::ryz, kR = B3Region<TANGENT>::r;
+  constexpr int kH = B3Region<T, TANGENT>::h;
+  constexpr int kRZ = B3Region<T, TANGENT>::rz, kRYZ = B3Region<T, TANGENT>::ryz, kR = B3Region<T, TANGENT>::r;
   // chain of this workgroup and its tile number within that chain's grid
   const int chain = (int)blockIdx.x >= nb ? 1 : 0;
   const int bl = (int)blockIdx.x - chain * nb;
@@ -321,7 +322,7 @@ k_block3(DevLattice L, B3Launch<T> la, int nb) {
 
 template <typename T, bool TANGENT>
 static size_t b3_lds() {
-  return (TANGENT ? 6 : 3) * (size_t)B3Region<TANGENT>::r * sizeof(T);
+  return (TANGENT ? 6 : 3) * (size_t)B3Region<T, TANGENT>::r * sizeof(T);
 }
 
 template <typename T, bool TG, bool SC, bool CN>
@@ -353,8 +354,8 @@ void launch_block3(const DevLattice& L, const B3Launch<T>& a, bool tangent, bool
                    hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
   SFX_REQUIRE(a.nchains == 1 || a.nchains == 2, SFX_E_INVALID, "a blocked launch advances one or two chains");
   for (int c = 0; c < a.nchains; c++)
-    SFX_REQUIRE(a.a[c].nr >= 1 && a.a[c].nr <= (tangent ? kB3TanRanks : kB3Ranks), SFX_E_INVALID,
-                "blocked launch of 1..4 ranks (1..2 for tangents)");
+    SFX_REQUIRE(a.a[c].nr >= 1 && a.a[c].nr <= b3_max_ranks<T>(tangent), SFX_E_INVALID,
+                "blocked launch of 1..4 ranks (1..2 for tangents and FP64)");
   SFX_REQUIRE(!(tangent && sizeof(T) == 8), SFX_E_UNSUPPORTED, "FP64 tangent sweeps run rank by rank");
   const size_t lds_base = b3_lds<T, false>(), lds_tan = b3_lds<T, true>();
   SFX_REQUIRE((lds_base <= 160 * 1024 && (sizeof(T) == 8 || lds_tan <= 160 * 1024)), SFX_E_UNSUPPORTED,

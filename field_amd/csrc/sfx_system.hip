@@ -622,6 +622,7 @@ void System::sweep(int m, bool scaled) {
 // zeroing of the accumulators and of the scale maxima has been done by sweep().
 template <typename T>
 void System::sweep_blocked(int m, bool scaled) {
+  constexpr int grid = b3_max_ranks<T>(false);  // ranks per launch at most
   const DevLattice& L = lat.d;
   const long long Mn = L.M;
   Molecule& mo = mols[m];
@@ -705,8 +706,8 @@ void System::sweep_blocked(int m, bool scaled) {
     auto bstate = [&](int k) { return mo.seq[rank_of(k) - 1]; };
     launch_init_rank2<T>(L, InitRank<T>{gp(mo.seq[0]), slot(0), fsc(1, 0)},
                          InitRank<T>{gp(mo.seq[N - 1]), slot(N - 1), bsc(1, N)}, st);
-    const auto F1 = chain_launches(2, h, 2, state_of_rank);
-    const auto B1 = chain_launches(2, N - h, 2, bstate);
+    const auto F1 = chain_launches(2, h, 2, state_of_rank, grid);
+    const auto B1 = chain_launches(2, N - h, 2, bstate, grid);
     for (size_t i = 0; i < std::max(F1.size(), B1.size()); i++) {
       B3Launch<T> la{};
       if (i < F1.size()) {
@@ -726,8 +727,8 @@ void System::sweep_blocked(int m, bool scaled) {
       go(la, false);
     }
     launch_set_ref_pair(scal, scaled ? lnf : nullptr, h, lnb, N - h, st);
-    const auto F2 = chain_launches(h + 1, N, h + 1, state_of_rank);
-    const auto B2 = chain_launches(N - h + 1, N, N - h + 1, bstate);
+    const auto F2 = chain_launches(h + 1, N, h + 1, state_of_rank, grid);
+    const auto B2 = chain_launches(N - h + 1, N, N - h + 1, bstate, grid);
     int fc = 0, bc = 0;
     for (size_t i = 0; i < std::max(F2.size(), B2.size()); i++) {
       B3Launch<T> la{};
@@ -790,7 +791,7 @@ void System::sweep_blocked(int m, bool scaled) {
       launch(sa, sb, slot(prev), mo.seq[sa - 1], maxf, lnf, outs, nullptr, nullptr, false);
       if (plan.c && last == plan.seg(parity)) parity ^= 1;
       prev = last;
-    });
+    }, grid);
   }
   // slot of forward rank s for requests in non-increasing s; recomputes a
   // checkpoint segment with the forward pass's own launch partition
@@ -807,7 +808,7 @@ void System::sweep_blocked(int m, bool scaled) {
         for (int s2 = sa; s2 <= sb; s2++) outs[s2 - sa] = slot(plan.seg_in(set, s2 - start - 1));
         launch(sa, sb, slot(prev), mo.seq[sa - 1], maxf, lnf, outs, nullptr, nullptr, false);
         prev = plan.seg_in(set, sb - start - 1);
-      });
+      }, grid);
       resident[set] = j;
     }
     return plan.seg_in(set, s - start - 1);
@@ -835,7 +836,7 @@ void System::sweep_blocked(int m, bool scaled) {
           outs[kb - ka] = pp[cur ^ 1];
           launch(ka, kb, pp[cur], mo.seq[rank_of(ka) - 1], maxb, lnb, outs, pairs, pidx, true);
           cur ^= 1;
-        });
+        }, grid);
     final_G = pp[cur];
     lnfinal = scaled ? lnb + N : nullptr;
   } else {
@@ -863,7 +864,7 @@ void System::sweep_blocked(int m, bool scaled) {
           outs[sb - sa] = pp[cur ^ 1];
           launch(sa, sb, pp[cur], mo.seq[sa - 1], maxf, lnf, outs, pairs, pidx, true);
           cur ^= 1;
-        });
+        }, grid);
     final_G = pp[cur];
     lnfinal = scaled ? lnf + N : nullptr;
   }
